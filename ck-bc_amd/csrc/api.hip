@@ -1,0 +1,281 @@
+// C ABI of the engine context (one device graph, plain and batched
+// workspaces sharing one BC accumulator).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "bcmi.h"
+#include "bcmi_internal.h"
+#include "engine.h"
+
+namespace bcmi {
+const char* last_error();
+}
+using namespace bcmi;
+
+struct bcmi_ctx {
+  int device = 0;
+  Params prm;
+  DevGraph g;
+  bool have_graph = false;
+  Workspace* wsp = nullptr;  // plain
+  Workspace* wsb = nullptr;  // batched
+  double* bc = nullptr;
+  hipStream_t st = nullptr;
+  RunResult last;
+  std::mutex mu;
+};
+
+#define CHECK_CTX(c) \
+  if (!(c)) return set_error(BCMI_ERR_ARG, "null context")
+
+static int use_device(bcmi_ctx* c) {
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  return BCMI_OK;
+}
+
+static void drop_workspaces(bcmi_ctx* c) {
+  workspace_destroy(c->wsp);
+  workspace_destroy(c->wsb);
+  c->wsp = c->wsb = nullptr;
+}
+
+static int get_ws(bcmi_ctx* c, int mode, Workspace** out) {
+  Workspace*& ws = mode == BCMI_MODE_PLAIN ? c->wsp : c->wsb;
+  if (!ws) {
+    int rc = BCMI_OK;
+    ws = workspace_create(c->g, mode, c->prm.width, c->prm, &rc);
+    if (!ws) return rc;
+    workspace_share_bc(ws, c->bc);
+  }
+  *out = ws;
+  return BCMI_OK;
+}
+
+extern "C" {
+
+const char* bcmi_last_error(void) { return last_error(); }
+int bcmi_version(void) { return 1; }
+
+int bcmi_create(int device, bcmi_ctx** out) {
+  if (!out) return set_error(BCMI_ERR_ARG, "null output");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return set_error(BCMI_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return set_error(BCMI_ERR_ARG, "device index out of range");
+  bcmi_ctx* c = new bcmi_ctx();
+  c->device = device;
+  int rc = use_device(c);
+  if (rc) { delete c; return rc; }
+  e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return set_error(BCMI_ERR_HIP, "stream creation failed"); }
+  *out = c;
+  return BCMI_OK;
+}
+
+int bcmi_destroy(bcmi_ctx* c) {
+  CHECK_CTX(c);
+  use_device(c);
+  drop_workspaces(c);
+  c->g.release();
+  if (c->bc) (void)hipFree(c->bc);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return BCMI_OK;
+}
+
+int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
+  CHECK_CTX(c);
+  if (!key) return set_error(BCMI_ERR_ARG, "null key");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Params& p = c->prm;
+  std::string k(key);
+  if (k == "block") {
+    if (v < 64 || v > 256 || v % 64) return set_error(BCMI_ERR_ARG, "block must be 64, 128 or 256");
+    p.block = (int)v;
+  } else if (k == "grid") {
+    if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "grid must be in [1,65536]");
+    p.grid = (int)v;
+  } else if (k == "chunk") {
+    if (v < 64 || v > (1 << 20) || v % 64) return set_error(BCMI_ERR_ARG, "chunk must be a multiple of 64");
+    if (c->have_graph) return set_error(BCMI_ERR_STATE, "chunk must be set before uploading a graph");
+    p.chunk = (int)v;
+  } else if (k == "alpha_plain") {
+    if (v < 0) return set_error(BCMI_ERR_ARG, "alpha must be >= 0");
+    p.alpha_plain = v;
+  } else if (k == "alpha_batched") {
+    if (v < 0) return set_error(BCMI_ERR_ARG, "alpha must be >= 0");
+    p.alpha_batched = v;
+  } else if (k == "steps") {
+    if (v < 1 || v > 4096) return set_error(BCMI_ERR_ARG, "steps must be in [1,4096]");
+    p.steps = (int)v;
+  } else if (k == "width") {
+    if (v < 1 || v > 64) return set_error(BCMI_ERR_ARG, "width must be in [1,64]");
+    p.width = (int)v;
+  } else if (k == "ktime") {
+    if (v < 0 || v > 7) return set_error(BCMI_ERR_ARG, "ktime must be in [0,7]");
+    p.ktime = (int)v;
+  } else {
+    return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
+  }
+  return BCMI_OK;
+}
+
+int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
+  CHECK_CTX(c);
+  if (!key || !v) return set_error(BCMI_ERR_ARG, "null argument");
+  std::string k(key);
+  const Params& p = c->prm;
+  if (k == "block") *v = p.block;
+  else if (k == "grid") *v = p.grid;
+  else if (k == "chunk") *v = p.chunk;
+  else if (k == "alpha_plain") *v = p.alpha_plain;
+  else if (k == "alpha_batched") *v = p.alpha_batched;
+  else if (k == "steps") *v = p.steps;
+  else if (k == "width") *v = p.width;
+  else if (k == "ktime") *v = p.ktime;
+  else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
+  return BCMI_OK;
+}
+
+int bcmi_upload_graph(bcmi_ctx* c, int64_t n, const int64_t* off, const int32_t* adj, int directed, int heuristic) {
+  CHECK_CTX(c);
+  if (!off || (!adj && n > 0 && off[n] > 0)) return set_error(BCMI_ERR_ARG, "null CSR arrays");
+  if (heuristic != BCMI_H_NONE && heuristic != BCMI_H_PEEL) return set_error(BCMI_ERR_ARG, "heuristic must be 0 or 1");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = use_device(c);
+  if (rc) return rc;
+  drop_workspaces(c);
+  if (c->bc) { (void)hipFree(c->bc); c->bc = nullptr; }
+  c->have_graph = false;
+  rc = devgraph_build(c->g, n, off, adj, directed, heuristic, c->prm, c->st);
+  if (rc) { c->g.release(); return rc; }
+  hipError_t e = hipMalloc((void**)&c->bc, sizeof(double) * n);
+  if (e != hipSuccess) { c->g.release(); return set_error(BCMI_ERR_NOMEM, "BC accumulator allocation failed"); }
+  e = hipMemsetAsync(c->bc, 0, sizeof(double) * n, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  c->have_graph = true;
+  return BCMI_OK;
+}
+
+int bcmi_get_peel(bcmi_ctx* c, int32_t* round, int32_t* parent, uint64_t* w, int32_t* nrounds) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  use_device(c);
+  int64_t n = c->g.n;
+  hipError_t e = hipSuccess;
+  if (round && e == hipSuccess) e = hipMemcpy(round, c->g.round, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+  if (parent && e == hipSuccess) e = hipMemcpy(parent, c->g.parent, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+  if (w && e == hipSuccess) e = hipMemcpy(w, c->g.w, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  if (nrounds) *nrounds = c->g.nrounds;
+  return BCMI_OK;
+}
+
+int bcmi_get_labels(bcmi_ctx* c, int32_t* label) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  if (!label) return set_error(BCMI_ERR_ARG, "null output");
+  use_device(c);
+  hipError_t e = hipMemcpy(label, c->g.label, sizeof(int32_t) * c->g.n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  return BCMI_OK;
+}
+
+int bcmi_bc_reset(bcmi_ctx* c) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  use_device(c);
+  hipError_t e = hipMemsetAsync(c->bc, 0, sizeof(double) * c->g.n, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  return BCMI_OK;
+}
+
+int bcmi_bc_run(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int mode, int src_flags, bcmi_run_stats* st) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  if (nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad source list");
+  if (mode != BCMI_MODE_PLAIN && mode != BCMI_MODE_BATCHED) return set_error(BCMI_ERR_ARG, "bad mode");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = use_device(c);
+  if (rc) return rc;
+  Workspace* ws = nullptr;
+  rc = get_ws(c, mode, &ws);
+  if (rc) return rc;
+  RunResult R;
+  rc = bc_run(c->g, ws, c->prm, sources, nsrc, mode, src_flags, &R);
+  c->last = R;
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    st->sources_run = R.sweeps;
+    st->batches = R.launches;
+    st->te = R.te;
+    st->bytes = R.bytes;
+    st->gpu_ms = R.gpu_ms;
+    st->wall_ms = R.wall_ms;
+    st->max_levels = R.max_levels;
+    st->sum_n = R.sum_n;
+    st->sum_m = R.sum_m;
+    st->sum_a = R.sum_a;
+  }
+  return rc;
+}
+
+int bcmi_get_bc(bcmi_ctx* c, double* bc) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  if (!bc) return set_error(BCMI_ERR_ARG, "null output");
+  use_device(c);
+  hipError_t e = hipMemcpy(bc, c->bc, sizeof(double) * c->g.n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  return BCMI_OK;
+}
+
+int bcmi_get_sweep_stats(bcmi_ctx* c, int64_t* n_s, int64_t* m_s, int64_t* a_s, int32_t* levels) {
+  CHECK_CTX(c);
+  const RunResult& R = c->last;
+  size_t k = R.n_s.size();
+  if (n_s) memcpy(n_s, R.n_s.data(), k * sizeof(int64_t));
+  if (m_s) memcpy(m_s, R.m_s.data(), k * sizeof(int64_t));
+  if (a_s) memcpy(a_s, R.a_s.data(), k * sizeof(int64_t));
+  if (levels) memcpy(levels, R.levels.data(), R.levels.size() * sizeof(int32_t));
+  return BCMI_OK;
+}
+
+int bcmi_get_kernel_time(bcmi_ctx* c, double* ms, int64_t* launches) {
+  CHECK_CTX(c);
+  if (ms) *ms = c->last.ktime_ms;
+  if (launches) *launches = c->last.klaunches;
+  return BCMI_OK;
+}
+
+int bcmi_sssp(bcmi_ctx* c, int32_t s, int32_t* depth, uint64_t* sigma, double* delta) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  if (!depth || !sigma || !delta) return set_error(BCMI_ERR_ARG, "null output");
+  std::lock_guard<std::mutex> lk(c->mu);
+  use_device(c);
+  Workspace* ws = nullptr;
+  int rc = get_ws(c, BCMI_MODE_PLAIN, &ws);
+  if (rc) return rc;
+  return sssp_debug(c->g, ws, c->prm, s, depth, sigma, delta);
+}
+
+int bcmi_msbfs(bcmi_ctx* c, const int32_t* sources, int k, int32_t* depth, uint64_t* sigma, double* delta) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  if (!sources || !depth || !sigma || !delta) return set_error(BCMI_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  use_device(c);
+  Workspace* ws = nullptr;
+  int rc = get_ws(c, BCMI_MODE_BATCHED, &ws);
+  if (rc) return rc;
+  return msbfs_debug(c->g, ws, c->prm, sources, k, depth, sigma, delta);
+}
+
+}  // extern "C"

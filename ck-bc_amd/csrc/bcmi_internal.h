@@ -1,0 +1,32 @@
+// Internal declarations shared by the host graph code, the device engine and
+// the query server.
+#pragma once
+#include <cstdint>
+#include <string>
+
+#include "bcmi.h"
+
+namespace bcmi {
+
+// Records the message for bcmi_last_error() (thread-local) and returns code.
+int set_error(int code, const std::string& msg);
+
+bcmi_graph* graph_alloc(int64_t n, int64_t m, int directed);
+bcmi_graph* csr_from_arcs(int64_t n, const int32_t* tail, const int32_t* head, int64_t k, int mirror,
+                          int directed);
+
+// Tunable launch and algorithm parameters.
+struct Params {
+  int block = 256;          // threads per block (multiple of 64)
+  int grid = 2048;          // persistent grid (blocks)
+  int chunk = 256;          // max edges per work item
+  int64_t alpha_plain = 1000;   // x1000; bottom-up when frontier arcs > alpha * unvisited arcs
+  int64_t alpha_batched = 1000; // x1000
+  int steps = 16;           // levels per captured graph
+  int width = 64;           // sources per batch
+  int ktime = 0;            // kernel timed with HIP events inside the level loop:
+                            // 0 none, 1 batched sigma, 2 batched backward, 3 batched top-down,
+                            // 4 batched bottom-up, 5 plain top-down, 6 plain bottom-up, 7 plain backward
+};
+
+}  // namespace bcmi

@@ -1,0 +1,116 @@
+// Host-side view of the device engine: device-resident graph, per-slot
+// workspaces and the sweep drivers.  Shared by the context C ABI and the
+// query server.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "bcmi_internal.h"
+
+namespace bcmi {
+
+// Device control block of one workspace (one sweep in flight at a time).
+struct Ctl {
+  int32_t level;      // forward level being expanded
+  int32_t started;
+  int32_t fdone;      // forward finished
+  int32_t last;       // deepest non-empty level
+  int32_t blevel;     // backward level being processed
+  int32_t bstarted;
+  int32_t bdone;
+  int32_t mode;       // 0 top-down, 1 bottom-up
+  int32_t overflow;   // record capacity exceeded
+  int32_t sigma_ovf;  // a path count wrapped past 2^64-1
+  int32_t src_label;  // component of the plain-sweep source
+  int32_t width;      // sources in the batch
+  int64_t rec_count;  // records appended so far
+  int64_t arcs_next;  // degree sum of vertices appended during this step
+  int64_t verts_next;
+  int64_t arcs_cur;   // degree sum of the current frontier
+  int64_t verts_cur;
+  int64_t arcs_seen;  // plain: degree sum of visited vertices; batched: of complete vertices
+  int64_t arcs_reach; // degree sum of the source components
+  int64_t sum_a;      // shortest-path DAG arcs (plain sweep)
+  int64_t n_td, n_bu; // levels run top-down / bottom-up
+  int64_t tot_verts;  // plain: reached vertices; batched: (vertex, level) pairs
+  int64_t tot_arcs;   // degree sum over the same
+  int64_t uni_verts;  // batched: distinct vertices reached by any source
+  int64_t uni_arcs;   // batched: their degree sum
+};
+
+struct DevGraph {
+  int64_t n = 0;
+  int64_t m = 0;             // arcs of the traversed (core) graph
+  int64_t fm = 0;            // arcs of the uploaded graph
+  int directed = 0;
+  int heuristic = 0;
+  int chunk = 256;
+  int64_t* off = nullptr;    // traversed CSR
+  int32_t* adj = nullptr;
+  int64_t* toff = nullptr;   // transposed traversed CSR (aliases off/adj when undirected)
+  int32_t* tadj = nullptr;
+  int64_t* foff = nullptr;   // uploaded CSR (aliases off/adj when nothing was peeled)
+  int32_t* fadj = nullptr;
+  int32_t* label = nullptr;     // component label (smallest id) per vertex
+  int64_t* comp_size = nullptr; // vertices per label
+  int64_t* comp_arcs = nullptr; // traversed arcs per label
+  int32_t* all_v = nullptr;     // chunk records of every traversed vertex
+  int32_t* all_j = nullptr;
+  int64_t nall = 0;
+  // peeling
+  int32_t nrounds = 0;
+  int32_t* round = nullptr;
+  int32_t* parent = nullptr;
+  uint64_t* w = nullptr;
+  int32_t* root = nullptr;
+  int32_t* order = nullptr;      // removed vertices by round
+  std::vector<int64_t> round_lo; // offsets of each round in order (size nrounds+1)
+  std::vector<int64_t> h_off;    // traversed offsets on the host (degrees)
+  std::vector<int32_t> h_round;  // removal round per vertex on the host
+  size_t bytes = 0;
+  void release();
+};
+
+struct Workspace;
+
+// Uploads and prepares a graph (peeling, components, work lists).
+int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* adj, int directed,
+                   int heuristic, const Params& prm, hipStream_t st);
+
+// Bytes a workspace for graph g needs in the given mode.
+size_t workspace_bytes(const DevGraph& g, int mode, int width, const Params& prm);
+
+struct RunResult {
+  int64_t sweeps = 0, launches = 0, max_levels = 0;
+  int64_t sum_n = 0, sum_m = 0, sum_a = 0;
+  int64_t n_td = 0, n_bu = 0;
+  std::vector<int64_t> uni_n, uni_m;  // batched: union reach per batch
+  double te = 0, bytes = 0, gpu_ms = 0, wall_ms = 0;
+  double ktime_ms = 0;      // summed duration of the timed kernel (Params::ktime)
+  int64_t klaunches = 0;    // its launches (including ones that found no work)
+  std::vector<int64_t> n_s, m_s, a_s;
+  std::vector<int32_t> levels;
+};
+
+Workspace* workspace_create(const DevGraph& g, int mode, int width, const Params& prm, int* rc);
+void workspace_destroy(Workspace* ws);
+hipStream_t workspace_stream(Workspace* ws);
+double* workspace_bc(Workspace* ws);
+// Makes the workspace accumulate into an external BC array (not owned).
+void workspace_share_bc(Workspace* ws, double* bc);
+size_t workspace_size(Workspace* ws);
+int workspace_bc_reset(Workspace* ws);
+
+// Accumulates BC over the sources (multiset) into the workspace BC array.
+int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* sources, int64_t nsrc,
+           int mode, int src_flags, RunResult* out);
+
+// Debug sweeps (no BC accumulation).
+int sssp_debug(const DevGraph& g, Workspace* ws, const Params& prm, int32_t s, int32_t* depth,
+               uint64_t* sigma, double* delta);
+int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* src, int k,
+                int32_t* depth, uint64_t* sigma, double* delta);
+
+}  // namespace bcmi

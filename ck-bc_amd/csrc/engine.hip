@@ -1,0 +1,1873 @@
+// Brandes betweenness centrality on MI355X (gfx950): device kernels and the
+// sweep drivers.
+//
+// Layout in HBM (n vertices, m arcs of the traversed graph):
+//   CSR        off int64[n+1], adj int32[m]              (one block, the 1x1 case of
+//                                                          the reference's 2D grid, bc2d.h:28-66)
+//   plain      depth int32[n], sigma uint64[n], delta f64[n], coef f64[n]
+//   batched    seen/lm0/lm1/lm2 uint64[n] (one bit per source of the batch),
+//              sigma uint64[n][64], coef f64[n][64]   (source-minor rows: a wave's
+//              64 lanes are the 64 sources, one 512-byte row per vertex)
+//   work lists chunk records (vertex, chunk index) of every BFS level, appended
+//              level after level like the reference's concatenated frontier
+//              queues (cuda_kernels.cu:910-913), each covering at most `chunk`
+//              arcs of its vertex so that hubs are split across waves.
+//
+// Reference rules restated (paths relative to the ck-bc root, "cu" =
+// program/bc-cuda/cuda_kernels.cu):
+//   forward  cu:524-537  first visit fixes the depth; sigma[r] += sigma[c] for
+//                        every arc c->r with depth[r] == depth[c]+1
+//   backward cu:590-596  delta[c] += (delta[r]+1)/sigma[r]*sigma[c] for every
+//                        arc c->r with depth[r] == depth[c]+1, depths D-2..1
+//   reset    cu:1089-1119 delta starts at the pruned-subtree weight w(v)
+//   update   cu:263-277  bc[v] += delta[v] * multiplicity of the source
+//
+// Every level is one step of a device-driven loop: a one-thread "decide"
+// kernel closes the level's work list and picks top-down or bottom-up from
+// device counters; the expansion kernels are persistent (fixed grid, wave
+// strided over work items) and return at once when their direction was not
+// chosen or the sweep has finished.  A hipGraph holds `steps` such levels
+// and is replayed until the control block reports completion, so the host
+// neither sizes launches nor reads counters per level (the reference does
+// three device-to-host copies and two syncs per level, cu:861-907).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bcmi.h"
+#include "bcmi_internal.h"
+#include "engine.h"
+
+namespace bcmi {
+
+static thread_local std::string g_err;
+int set_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+const char* last_error() { return g_err.c_str(); }
+
+#define HIP_TRY(call)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return set_error(e_ == hipErrorOutOfMemory ? BCMI_ERR_NOMEM : BCMI_ERR_HIP,                  \
+                       std::string(#call) + ": " + hipGetErrorString(e_));                        \
+  } while (0)
+
+typedef unsigned long long u64;
+
+// ------------------------------------------------------------ wave helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ u64 rl64(u64 x, int l) {
+  unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, l);
+  unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
+  return ((u64)hi << 32) | lo;
+}
+__device__ __forceinline__ int rl32(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+
+__device__ __forceinline__ int wave_excl_scan(int x, int* total) {
+  int lane = lane_id();
+  int inc = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  *total = __shfl(inc, 63, 64);
+  return inc - x;
+}
+__device__ __forceinline__ double wave_sum_d(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+// Sum with wrap detection (sets *ovf when any partial sum wraps).
+__device__ __forceinline__ u64 wave_sum_u64_ovf(u64 x, int* ovf) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    u64 y = __shfl_xor(x, o, 64);
+    u64 s = x + y;
+    if (s < x) *ovf = 1;
+    x = s;
+  }
+  return x;
+}
+__device__ __forceinline__ u64 wave_or64(u64 x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 ld_coherent_u64(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int nchunks(int64_t deg, int chunk) {
+  return deg <= chunk ? 1 : (int)((deg + chunk - 1) / chunk);
+}
+
+// Device-side kernel timer: first block entry to last block exit of every
+// launch, on the constant-rate wall clock (s_memrealtime), summed on device.
+struct KT {
+  u64 start, end, total, launches;
+  unsigned count;
+};
+__device__ __forceinline__ void kt_enter(KT* kt) {
+  if (kt && threadIdx.x == 0) atomicMin(&kt->start, (u64)wall_clock64());
+}
+__device__ __forceinline__ void kt_exit(KT* kt) {
+  if (!kt) return;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicMax(&kt->end, (u64)wall_clock64());
+    unsigned prev = atomicAdd(&kt->count, 1u);
+    if (prev == gridDim.x - 1) {
+      __threadfence();
+      u64 s0 = ld_coherent_u64(&kt->start), e0 = ld_coherent_u64(&kt->end);
+      kt->total += e0 - s0;
+      kt->launches += 1;
+      kt->start = ~0ull;
+      kt->end = 0;
+      kt->count = 0;
+    }
+  }
+}
+
+// ------------------------------------------------------------ kernel args
+struct GA {
+  const int64_t* off;
+  const int32_t* adj;
+  const int64_t* toff;  // parents of v: v appears in adj of each (transpose; == off/adj if undirected)
+  const int32_t* tadj;
+  int directed;
+  const int32_t* label;
+  const int64_t* comp_arcs;
+  const int32_t* all_v;
+  const int32_t* all_j;
+  int64_t nall;
+  const uint64_t* w;
+  int chunk;
+  int64_t n;
+};
+
+struct PA {
+  Ctl* ctl;
+  int32_t* depth;
+  u64* sigma;
+  double* delta;
+  double* coef;
+  int32_t* done;
+  int32_t* rec_v;
+  int32_t* rec_j;
+  int64_t cap;
+  int64_t* rec_lo;
+  const int32_t* srcbuf;
+  const double* multbuf;
+  double* bc;
+  int64_t alpha;
+};
+
+struct BA {
+  Ctl* ctl;
+  u64* seen;
+  u64* lm;  // 3 arrays of n
+  u64* sigma;
+  double* coef;
+  int32_t* done;
+  int32_t* rec_v;
+  int32_t* rec_j;
+  u64* rec_mask;
+  int64_t cap;
+  int64_t* rec_lo;
+  u64* compmask;
+  const int32_t* srcbuf;
+  const double* multbuf;
+  double* bc;
+  u64* st_n;
+  u64* st_m;
+  u64* st_a;
+  double* dbg_delta;
+  int64_t alpha;
+  int64_t n;
+};
+
+// Appends the vertices of the lanes with `want` set to the work list: each
+// gets ceil(deg/chunk) records.  Wave-wide call (every lane must call it).
+__device__ __forceinline__ int64_t wave_append(bool want, int32_t v, int64_t degv, int chunk, Ctl* c,
+                                               int32_t* rec_v, int32_t* rec_j, int64_t cap) {
+  int nch = want ? nchunks(degv, chunk) : 0;
+  int tot;
+  int pre = wave_excl_scan(nch, &tot);
+  if (tot == 0) return -1;
+  int lane = lane_id();
+  long long dsum = wave_sum_ll(want ? degv : 0);
+  int nv = __popcll(__ballot(want));
+  u64 base = 0;
+  if (lane == 0) {
+    base = atomicAdd((u64*)&c->rec_count, (u64)tot);
+    atomicAdd((u64*)&c->arcs_next, (u64)dsum);
+    atomicAdd((u64*)&c->verts_next, (u64)nv);
+  }
+  base = __shfl(base, 0, 64);
+  int64_t my = (int64_t)base + pre;
+  for (int k = 0; k < nch; k++) {
+    int64_t r = my + k;
+    if (r < cap) {
+      rec_v[r] = v;
+      rec_j[r] = k;
+    } else {
+      c->overflow = 1;
+    }
+  }
+  return want ? my : -1;
+}
+
+// ============================================================ plain sweep
+__global__ void k_plain_init(GA g, PA p) {
+  Ctl* c = p.ctl;
+  int32_t s = p.srcbuf[0];
+  int64_t deg = g.off[s + 1] - g.off[s];
+  int nch = nchunks(deg, g.chunk);
+  c->level = 0; c->started = 0; c->fdone = 0; c->last = 0;
+  c->blevel = 0; c->bstarted = 0; c->bdone = 0; c->mode = 0;
+  c->overflow = 0;
+  c->src_label = g.label[s];
+  c->rec_count = nch;
+  c->arcs_next = deg; c->verts_next = 1; c->arcs_cur = 0; c->verts_cur = 0;
+  c->arcs_seen = 0;
+  c->arcs_reach = g.comp_arcs[g.label[s]];
+  c->sum_a = 0; c->n_td = 0; c->n_bu = 0;
+  c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 1; c->uni_arcs = deg;
+  c->sigma_ovf = 0;
+  p.rec_lo[0] = 0;
+  for (int k = 0; k < nch && k < p.cap; k++) { p.rec_v[k] = s; p.rec_j[k] = k; }
+  if (nch > p.cap) c->overflow = 1;
+  p.depth[s] = 0;
+  p.sigma[s] = 1;
+  p.delta[s] = g.w ? (double)g.w[s] : 0.0;
+}
+
+__device__ __forceinline__ bool decide_common(Ctl* c, int64_t* rec_lo, int64_t alpha) {
+  if (c->fdone) return false;
+  if (c->started) c->level++;
+  c->started = 1;
+  int d = c->level;
+  rec_lo[d + 1] = c->rec_count;
+  if (rec_lo[d + 1] == rec_lo[d] || c->overflow) {
+    c->fdone = 1;
+    c->last = d - 1;
+    return false;
+  }
+  c->arcs_cur = c->arcs_next;
+  c->verts_cur = c->verts_next;
+  c->tot_arcs += c->arcs_cur;
+  c->tot_verts += c->verts_cur;
+  c->arcs_next = 0;
+  c->verts_next = 0;
+  return true;
+}
+
+__global__ void k_plain_decide(GA g, PA p) {
+  Ctl* c = p.ctl;
+  if (!decide_common(c, p.rec_lo, p.alpha)) return;
+  // Top-down reads the frontier's arcs; bottom-up reads the arcs of every
+  // unvisited vertex of the component (no early exit: BC needs all parents).
+  c->arcs_seen += c->arcs_cur;
+  int64_t unvisited = c->arcs_reach - c->arcs_seen;
+  bool bu = !g.directed && (double)c->arcs_cur * 1000.0 > (double)p.alpha * (double)unvisited;
+  c->mode = bu ? 1 : 0;
+  if (bu) c->n_bu++; else c->n_td++;
+}
+
+// Top-down: every arc u->v out of level d; first visitor of v claims it with
+// a CAS on depth and appends it; sigma[v] += sigma[u] when depth[v] == d+1.
+__device__ __forceinline__ void k_plain_td_body(GA g, PA p) {
+  Ctl* c = p.ctl;
+  if (c->fdone || c->mode != 0) return;
+  const int d = c->level;
+  const int64_t lo = p.rec_lo[d], hi = p.rec_lo[d + 1];
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  long long acnt = 0;
+  int ovf = 0;
+  for (int64_t r = lo + gw; r < hi; r += tw) {
+    int32_t u = p.rec_v[r];
+    int j = p.rec_j[r];
+    int64_t e0 = g.off[u] + (int64_t)j * g.chunk;
+    int64_t e1 = min(g.off[u + 1], e0 + g.chunk);
+    u64 su = p.sigma[u];
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      int64_t e = eb + lane;
+      bool valid = e < e1;
+      int32_t v = valid ? g.adj[e] : 0;
+      bool won = false;
+      int64_t dg = 0;
+      if (valid) {
+        int dv = p.depth[v];
+        if (dv == -1) {
+          int old = atomicCAS(&p.depth[v], -1, d + 1);
+          won = (old == -1);
+          dv = won ? d + 1 : old;
+        }
+        if (dv == d + 1) {
+          u64 o = atomicAdd(&p.sigma[v], su);
+          if (o + su < o) ovf = 1;
+          acnt++;
+        }
+        if (won) {
+          p.delta[v] = g.w ? (double)g.w[v] : 0.0;
+          dg = g.off[v + 1] - g.off[v];
+        }
+      }
+      wave_append(won, v, dg, g.chunk, c, p.rec_v, p.rec_j, p.cap);
+    }
+  }
+  acnt = wave_sum_ll(acnt);
+  if (lane == 0 && acnt) atomicAdd((u64*)&c->sum_a, (u64)acnt);
+  if (ovf) c->sigma_ovf = 1;
+}
+
+__global__ void __launch_bounds__(256) k_plain_td(GA g, PA p, KT* kt) {
+  kt_enter(kt);
+  k_plain_td_body(g, p);
+  kt_exit(kt);
+}
+
+// Bottom-up: every unvisited vertex v of the source's component sums sigma
+// over its neighbours at depth d (all of them: no early exit).
+__device__ __forceinline__ void k_plain_bu_body(GA g, PA p) {
+  Ctl* c = p.ctl;
+  if (c->fdone || c->mode != 1) return;
+  const int d = c->level;
+  const int lab = c->src_label;
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  long long acnt = 0;
+  int ovf = 0;
+  for (int64_t r = gw; r < g.nall; r += tw) {
+    int32_t v = g.all_v[r];
+    if (g.label[v] != lab) continue;
+    int dv = p.depth[v];
+    if (dv != -1 && dv != d + 1) continue;
+    int j = g.all_j[r];
+    int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
+    int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+    u64 acc = 0;
+    int cnt = 0;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      int64_t e = eb + lane;
+      if (e < e1) {
+        int32_t u = g.adj[e];
+        if (p.depth[u] == d) {
+          u64 su = p.sigma[u];
+          u64 s = acc + su;
+          if (s < acc) ovf = 1;
+          acc = s;
+          cnt++;
+        }
+      }
+    }
+    acc = wave_sum_u64_ovf(acc, &ovf);
+    cnt = (int)wave_sum_ll(cnt);
+    bool won = false;
+    if (cnt > 0 && lane == 0) {
+      int old = atomicCAS(&p.depth[v], -1, d + 1);
+      won = (old == -1);
+      u64 o = atomicAdd(&p.sigma[v], acc);
+      if (o + acc < o) ovf = 1;
+      if (won) p.delta[v] = g.w ? (double)g.w[v] : 0.0;
+      acnt += cnt;
+    }
+    wave_append(won, v, won ? g.off[v + 1] - g.off[v] : 0, g.chunk, c, p.rec_v, p.rec_j, p.cap);
+  }
+  acnt = wave_sum_ll(acnt);
+  if (lane == 0 && acnt) atomicAdd((u64*)&c->sum_a, (u64)acnt);
+  if (ovf) c->sigma_ovf = 1;
+}
+
+__global__ void __launch_bounds__(256) k_plain_bu(GA g, PA p, KT* kt) {
+  kt_enter(kt);
+  k_plain_bu_body(g, p);
+  kt_exit(kt);
+}
+
+// Backward levels run from the deepest one (its vertices have no successors,
+// so delta = w(v)) up to depth 1; the source's own delta is never needed.
+__global__ void k_bdecide(Ctl* c) {
+  if (!c->fdone || c->bdone) return;
+  if (c->overflow) {
+    c->bdone = 1;
+    return;
+  }
+  if (!c->bstarted) {
+    c->blevel = c->last;
+    c->bstarted = 1;
+  } else {
+    c->blevel--;
+  }
+  if (c->blevel < 1) c->bdone = 1;
+}
+
+// Backward level d: delta[v] = w(v) + sum over successors z of
+// (delta[z]+1)/sigma[z]*sigma[v]; coef[z] = (delta[z]+1)/sigma[z] was stored
+// when z was finished.  Vertices split over several records combine their
+// partial sums with FP64 atomics; the last record finishes the vertex.
+__device__ __forceinline__ void k_plain_back_body(GA g, PA p) {
+  Ctl* c = p.ctl;
+  if (!c->fdone || c->bdone) return;
+  const int d = c->blevel;
+  const int64_t lo = p.rec_lo[d], hi = p.rec_lo[d + 1];
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  const double mult = p.multbuf[0];
+  for (int64_t r = lo + gw; r < hi; r += tw) {
+    int32_t v = p.rec_v[r];
+    int j = p.rec_j[r];
+    int64_t deg = g.off[v + 1] - g.off[v];
+    int nch = nchunks(deg, g.chunk);
+    int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
+    int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+    double sv = (double)p.sigma[v];
+    double acc = 0.0;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      int64_t e = eb + lane;
+      if (e < e1) {
+        int32_t z = g.adj[e];
+        if (p.depth[z] == d + 1) acc = acc + p.coef[z] * sv;
+      }
+    }
+    acc = wave_sum_d(acc);
+    if (lane == 0) {
+      double dl;
+      bool fin = true;
+      if (nch == 1) {
+        dl = p.delta[v] + acc;
+      } else {
+        atomicAdd(&p.delta[v], acc);
+        __threadfence();
+        int t = atomicAdd(&p.done[v], 1);
+        fin = (t == nch - 1);
+        if (fin) {
+          p.done[v] = 0;
+          dl = ld_coherent(&p.delta[v]);
+        }
+      }
+      if (fin) {
+        p.delta[v] = dl;
+        p.coef[v] = (dl + 1.0) / sv;
+        p.bc[v] += mult * dl;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_plain_back(GA g, PA p, KT* kt) {
+  kt_enter(kt);
+  k_plain_back_body(g, p);
+  kt_exit(kt);
+}
+
+__global__ void k_plain_cleanup(PA p) {
+  Ctl* c = p.ctl;
+  if (!c->fdone || !c->bdone) return;
+  int64_t n = c->rec_count < p.cap ? c->rec_count : p.cap;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    if (p.rec_j[r] == 0) {
+      int32_t v = p.rec_v[r];
+      p.depth[v] = -1;
+      p.sigma[v] = 0;
+    }
+  }
+}
+
+// ========================================================== batched sweep
+// Sources of a batch are the lanes of a wave: bit i of a vertex mask and
+// column i of its sigma/coef row belong to source i.  Level lists hold
+// (vertex, mask of the sources that reach it at that depth).  Dense masks:
+// seen (sources that reached v so far) and three rotating level arrays lm[k]
+// with lm[d%3] = mask of sources for which v has depth d, kept exact for the
+// levels a kernel reads.
+__global__ void k_ms_init(GA g, BA b, int k) {
+  Ctl* c = b.ctl;
+  const int i = threadIdx.x;  // one block of 64 threads
+  u64* lm0 = b.lm;
+  if (i == 0) {
+    c->level = 0; c->started = 0; c->fdone = 0; c->last = 0;
+    c->blevel = 0; c->bstarted = 0; c->bdone = 0; c->mode = 0;
+    c->overflow = 0;
+    c->rec_count = 0;
+    c->arcs_next = 0; c->verts_next = 0; c->arcs_cur = 0; c->verts_cur = 0;
+    c->arcs_seen = 0; c->sum_a = 0; c->n_td = 0; c->n_bu = 0;
+    c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 0; c->uni_arcs = 0;
+    c->sigma_ovf = 0; c->width = k;
+    b.rec_lo[0] = 0;
+  }
+  b.st_n[i] = 0; b.st_m[i] = 0; b.st_a[i] = 0;
+  int32_t s = i < k ? b.srcbuf[i] : 0;
+  if (i < k) {
+    atomicOr(&lm0[s], 1ull << i);
+    atomicOr(&b.seen[s], 1ull << i);
+    atomicOr(&b.compmask[g.label[s]], 1ull << i);
+    b.sigma[(int64_t)s * 64 + i] = 1;
+  }
+  __syncthreads();
+  if (i < k) {
+    u64 msk = lm0[s];
+    int64_t deg = g.off[s + 1] - g.off[s];
+    b.st_n[i] = 1;
+    b.st_m[i] = deg;
+    if (__ffsll((long long)msk) - 1 == i) {  // first lane holding this vertex appends it
+      int nch = nchunks(deg, g.chunk);
+      u64 base = atomicAdd((u64*)&c->rec_count, (u64)nch);
+      atomicAdd((u64*)&c->arcs_next, (u64)deg);
+      atomicAdd((u64*)&c->verts_next, 1ull);
+      for (int q = 0; q < nch; q++) {
+        if ((int64_t)base + q < b.cap) {
+          b.rec_v[base + q] = s;
+          b.rec_j[base + q] = q;
+        } else {
+          c->overflow = 1;
+        }
+      }
+      if ((int64_t)base < b.cap) b.rec_mask[base] = msk;
+      atomicAdd((u64*)&c->uni_verts, 1ull);
+      atomicAdd((u64*)&c->uni_arcs, (u64)deg);
+    }
+  }
+  __syncthreads();
+  if (i == 0) {
+    // Degree sum of the components holding sources, and of the sources
+    // already complete (reached by every source of their component).
+    int64_t reach = 0, done = 0;
+    for (int q = 0; q < k; q++) {
+      int32_t sq = b.srcbuf[q];
+      int lq = g.label[sq];
+      bool first = true;
+      for (int t = 0; t < q; t++)
+        if (g.label[b.srcbuf[t]] == lq) { first = false; break; }
+      if (first) reach += g.comp_arcs[lq];
+      bool firstv = true;
+      for (int t = 0; t < q; t++)
+        if (b.srcbuf[t] == sq) { firstv = false; break; }
+      if (firstv && b.seen[sq] == b.compmask[lq]) done += g.off[sq + 1] - g.off[sq];
+    }
+    c->arcs_reach = reach;
+    c->arcs_seen = done;
+  }
+}
+
+__global__ void k_ms_decide(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (!decide_common(c, b.rec_lo, b.alpha)) return;
+  int64_t incomplete = c->arcs_reach - c->arcs_seen;
+  bool bu = !g.directed && (double)c->arcs_cur * 1000.0 > (double)b.alpha * (double)incomplete;
+  c->mode = bu ? 1 : 0;
+  if (bu) c->n_bu++; else c->n_td++;
+}
+
+// Zeroes the sigma columns of the sources that have not reached v yet, for
+// vertices split over several records (their records add into sigma
+// atomically).  Wave-wide.
+__device__ __forceinline__ void zero_multi_sigma(bool need, int32_t v, const BA& b) {
+  u64 pend = __ballot(need);
+  const int lane = lane_id();
+  while (pend) {
+    int q = __ffsll((long long)pend) - 1;
+    pend &= pend - 1;
+    int32_t vq = rl32(v, q);
+    u64 sn = b.seen[vq];
+    if (!((sn >> lane) & 1)) b.sigma[(int64_t)vq * 64 + lane] = 0;
+  }
+}
+
+// Top-down discovery: for every arc u->v out of level d, the sources of u
+// that have not reached v are OR-ed into the next-level mask of v; the lane
+// that turns that mask non-zero appends v.
+__device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (c->fdone || c->mode != 0) return;
+  const int d = c->level;
+  u64* nxt = b.lm + (int64_t)((d + 1) % 3) * b.n;
+  const int64_t lo = b.rec_lo[d], hi = b.rec_lo[d + 1];
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  for (int64_t r = lo + gw; r < hi; r += tw) {
+    int32_t u = b.rec_v[r];
+    int j = b.rec_j[r];
+    u64 F = b.rec_mask[r - j];
+    int64_t e0 = g.off[u] + (int64_t)j * g.chunk;
+    int64_t e1 = min(g.off[u + 1], e0 + g.chunk);
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      int64_t e = eb + lane;
+      bool valid = e < e1;
+      int32_t v = valid ? g.adj[e] : 0;
+      bool won = false;
+      int64_t dg = 0;
+      if (valid) {
+        u64 bits = F & ~b.seen[v];
+        if (bits && (nxt[v] & bits) != bits) {
+          u64 old = atomicOr(&nxt[v], bits);
+          won = (old == 0);
+          if (won) dg = g.off[v + 1] - g.off[v];
+        }
+      }
+      wave_append(won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
+      zero_multi_sigma(won && dg > g.chunk, v, b);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
+  kt_enter(kt);
+  k_ms_td_body(g, b);
+  kt_exit(kt);
+}
+
+// Bottom-up discovery: every vertex not yet reached by all sources of its
+// component ORs the level-d masks of its neighbours.
+__device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (c->fdone || c->mode != 1) return;
+  const int d = c->level;
+  const u64* cur = b.lm + (int64_t)(d % 3) * b.n;
+  u64* nxt = b.lm + (int64_t)((d + 1) % 3) * b.n;
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  for (int64_t r = gw; r < g.nall; r += tw) {
+    int32_t v = g.all_v[r];
+    u64 sv = b.seen[v];
+    u64 tgt = b.compmask[g.label[v]];
+    if (sv == tgt) continue;
+    int j = g.all_j[r];
+    int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
+    int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+    u64 acc = 0;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      int64_t e = eb + lane;
+      if (e < e1) acc |= cur[g.adj[e]];
+    }
+    acc = wave_or64(acc) & ~sv;
+    bool won = false;
+    int64_t dg = 0;
+    if (acc && lane == 0) {
+      if ((nxt[v] & acc) != acc) {
+        u64 old = atomicOr(&nxt[v], acc);
+        won = (old == 0);
+        if (won) dg = g.off[v + 1] - g.off[v];
+      }
+    }
+    wave_append(won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
+    zero_multi_sigma(won && dg > g.chunk, v, b);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
+  kt_enter(kt);
+  k_ms_bu_body(g, b);
+  kt_exit(kt);
+}
+
+// Path counts of level d+1: lane i of the wave sums sigma[u][i] over the
+// neighbours u that are at depth d for source i (the parents).  The first
+// record of v publishes its mask, marks it seen and counts statistics.  Also
+// clears level d-1 from its rotating array (not read at this step).
+__device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (c->fdone) return;
+  const int d = c->level;
+  const u64* cur = b.lm + (int64_t)(d % 3) * b.n;
+  const u64* nxt = b.lm + (int64_t)((d + 1) % 3) * b.n;
+  const int64_t lo = b.rec_lo[d + 1];
+  const int64_t hi = min(c->rec_count, b.cap);
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  u64 cn = 0, cm = 0, ca = 0;
+  int ovf = 0;
+  long long done_arcs = 0, uv = 0, ua = 0;
+  for (int64_t r = lo + gw; r < hi; r += tw) {
+    int32_t v = b.rec_v[r];
+    int j = b.rec_j[r];
+    u64 N = nxt[v];
+    int64_t deg = g.off[v + 1] - g.off[v];
+    int nch = nchunks(deg, g.chunk);
+    int64_t e0, e1;
+    if (!g.directed) {
+      e0 = g.off[v] + (int64_t)j * g.chunk;
+      e1 = min(g.off[v + 1], e0 + g.chunk);
+    } else {  // parents come from the transposed lists, split evenly over v's records
+      int64_t t0 = g.toff[v], tl = g.toff[v + 1] - t0;
+      e0 = t0 + tl * j / nch;
+      e1 = t0 + tl * (j + 1) / nch;
+    }
+    const bool mine = (N >> lane) & 1;
+    u64 acc = 0;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      int64_t e = eb + lane;
+      int32_t u = 0;
+      u64 F = 0;
+      if (e < e1) {
+        u = g.tadj[e];
+        F = cur[u] & N;
+      }
+      u64 P = __ballot(F != 0);
+      while (P) {
+        int q = __ffsll((long long)P) - 1;
+        P &= P - 1;
+        int32_t uq = rl32(u, q);
+        u64 fq = rl64(F, q);
+        if ((fq >> lane) & 1) {
+          u64 su = b.sigma[(int64_t)uq * 64 + lane];
+          u64 s = acc + su;
+          if (s < acc) ovf = 1;
+          acc = s;
+          ca++;
+        }
+      }
+    }
+    u64* row = b.sigma + (int64_t)v * 64 + lane;
+    if (nch == 1) {
+      if (mine) *row = acc;
+    } else if (mine) {
+      u64 o = atomicAdd(row, acc);
+      if (o + acc < o) ovf = 1;
+    }
+    if (j == 0) {
+      u64 so = b.seen[v];
+      u64 sn = so | N;
+      b.seen[v] = sn;
+      if (lane == 0 && so == 0) { uv++; ua += deg; }
+      if (lane == 0) b.rec_mask[r] = N;
+      if (mine) {
+        cn++;
+        cm += deg;
+        if (nch > 1) b.coef[(int64_t)v * 64 + lane] = 0.0;
+      }
+      if (lane == 0 && sn == b.compmask[g.label[v]]) done_arcs += deg;
+    }
+  }
+  // clear level d-1 (array (d+2)%3)
+  if (d >= 1) {
+    u64* old = b.lm + (int64_t)((d + 2) % 3) * b.n;
+    const int64_t plo = b.rec_lo[d - 1], phi = b.rec_lo[d];
+    for (int64_t r = plo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < phi;
+         r += (int64_t)gridDim.x * blockDim.x)
+      if (b.rec_j[r] == 0) old[b.rec_v[r]] = 0;
+  }
+  if (cn) atomicAdd(&b.st_n[lane], cn);
+  if (cm) atomicAdd(&b.st_m[lane], cm);
+  if (ca) atomicAdd(&b.st_a[lane], ca);
+  if (done_arcs) atomicAdd((u64*)&c->arcs_seen, (u64)done_arcs);
+  if (uv) atomicAdd((u64*)&c->uni_verts, (u64)uv);
+  if (ua) atomicAdd((u64*)&c->uni_arcs, (u64)ua);
+  if (ovf) c->sigma_ovf = 1;
+}
+
+__global__ void __launch_bounds__(256) k_ms_sigma(GA g, BA b, KT* kt) {
+  kt_enter(kt);
+  k_ms_sigma_body(g, b);
+  kt_exit(kt);
+}
+
+// Backward level d for a batch: lane i accumulates, in adjacency order,
+// coef[z][i] * sigma[v][i] over the successors z of v for source i
+// (neighbours with bit i in the level d+1 mask), starting from w(v).  Then
+// coef[v][i] = (delta+1)/sigma and bc[v] += sum_i mult_i * delta_i.  Also
+// publishes level d into lm[d%3] for the next (shallower) step and clears
+// level d+2 from lm[(d+2)%3].
+__device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (!c->fdone || c->bdone) return;
+  const int d = c->blevel;
+  const u64* lmn = b.lm + (int64_t)((d + 1) % 3) * b.n;
+  u64* lmd = b.lm + (int64_t)(d % 3) * b.n;
+  const int64_t lo = b.rec_lo[d], hi = b.rec_lo[d + 1];
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  const double mult = b.multbuf[lane];
+  for (int64_t r = lo + gw; r < hi; r += tw) {
+    int32_t v = b.rec_v[r];
+    int j = b.rec_j[r];
+    u64 L = b.rec_mask[r - j];
+    int64_t deg = g.off[v + 1] - g.off[v];
+    int nch = nchunks(deg, g.chunk);
+    int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
+    int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+    const bool mine = (L >> lane) & 1;
+    const double wv = g.w ? (double)g.w[v] : 0.0;
+    const double sv = mine ? (double)b.sigma[(int64_t)v * 64 + lane] : 1.0;
+    double acc = (nch == 1) ? wv : 0.0;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      int64_t e = eb + lane;
+      int32_t z = 0;
+      u64 M = 0;
+      if (e < e1) {
+        z = g.adj[e];
+        M = lmn[z] & L;
+      }
+      u64 P = __ballot(M != 0);
+      while (P) {
+        int q = __ffsll((long long)P) - 1;
+        P &= P - 1;
+        int32_t zq = rl32(z, q);
+        u64 mq = rl64(M, q);
+        if ((mq >> lane) & 1) acc = acc + b.coef[(int64_t)zq * 64 + lane] * sv;
+      }
+    }
+    double* crow = b.coef + (int64_t)v * 64 + lane;
+    bool fin = true;
+    double dl = acc;
+    if (nch > 1) {
+      if (mine) atomicAdd(crow, acc);
+      __threadfence();
+      int t = 0;
+      if (lane == 0) t = atomicAdd(&b.done[v], 1);
+      t = __shfl(t, 0, 64);
+      fin = (t == nch - 1);
+      if (fin) {
+        if (lane == 0) b.done[v] = 0;
+        dl = mine ? wv + ld_coherent(crow) : 0.0;
+      }
+    }
+    if (fin) {
+      double contrib = 0.0;
+      if (mine) {
+        *crow = (dl + 1.0) / sv;
+        contrib = mult * dl;
+        if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = dl;
+      }
+      contrib = wave_sum_d(contrib);
+      if (lane == 0) b.bc[v] += contrib;
+    }
+    if (j == 0 && lane == 0) lmd[v] = L;
+  }
+  {
+    u64* old = b.lm + (int64_t)((d + 2) % 3) * b.n;
+    const int64_t plo = b.rec_lo[d + 2], phi = b.rec_lo[d + 3];
+    if (d + 2 <= c->last)
+      for (int64_t r = plo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < phi;
+           r += (int64_t)gridDim.x * blockDim.x)
+        if (b.rec_j[r] == 0) old[b.rec_v[r]] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ms_back(GA g, BA b, KT* kt) {
+  kt_enter(kt);
+  k_ms_back_body(g, b);
+  kt_exit(kt);
+}
+
+__global__ void k_ms_cleanup(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (!c->fdone || !c->bdone) return;
+  int64_t n = c->rec_count < b.cap ? c->rec_count : b.cap;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    if (b.rec_j[r] == 0) {
+      int32_t v = b.rec_v[r];
+      b.seen[v] = 0;
+      b.lm[v] = 0;
+      b.lm[b.n + v] = 0;
+      b.lm[2 * b.n + v] = 0;
+      b.compmask[g.label[v]] = 0;
+    }
+  }
+}
+
+// ================================================= graph preparation kernels
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(u64* p, u64 v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f64(double* p, double v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_iota(int32_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = (int32_t)i;
+}
+
+// Union-find with lock-free hooking of the larger root under the smaller, so
+// the final label of a component is its smallest vertex id.
+__device__ int32_t uf_find(int32_t* par, int32_t x) {
+  for (;;) {
+    int32_t p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (p == x) return x;
+    int32_t gp = __hip_atomic_load(&par[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (gp != p) __hip_atomic_store(&par[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = gp;
+  }
+}
+__global__ void k_cc_union(const int64_t* off, const int32_t* adj, const int32_t* rv, const int32_t* rj,
+                           int64_t nrec, int chunk, int32_t* par) {
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  for (int64_t r = gw; r < nrec; r += tw) {
+    int32_t v = rv[r];
+    int j = rj[r];
+    int64_t e0 = off[v] + (int64_t)j * chunk;
+    int64_t e1 = min(off[v + 1], e0 + chunk);
+    for (int64_t e = e0 + lane; e < e1; e += 64) {
+      int32_t x = adj[e];
+      if (x <= v) continue;
+      int32_t a = v, bb = x;
+      for (;;) {
+        a = uf_find(par, a);
+        bb = uf_find(par, bb);
+        if (a == bb) break;
+        int32_t hi = a > bb ? a : bb, lo = a > bb ? bb : a;
+        if (atomicCAS(&par[hi], hi, lo) == hi) break;
+      }
+    }
+  }
+}
+__global__ void k_cc_finish(int32_t* par, int64_t n, const int64_t* coff, int64_t* csize, int64_t* carcs) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    int32_t l = uf_find(par, (int32_t)v);
+    par[v] = l;
+    atomicAdd((u64*)&csize[l], 1ull);
+    int64_t dg = coff[v + 1] - coff[v];
+    if (dg) atomicAdd((u64*)&carcs[l], (u64)dg);
+  }
+}
+
+// Peeling, round r.  Candidates: live vertices of live degree 1.
+__global__ void k_peel_first(const int64_t* off, int64_t n, int32_t* deg, int32_t* list, u64* cnt) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    int32_t dg = (int32_t)(off[v + 1] - off[v]);
+    deg[v] = dg;
+    if (dg == 1) list[atomicAdd(cnt, 1ull)] = (int32_t)v;
+  }
+}
+__global__ void k_peel_mark(const int32_t* list, int64_t nl, const int32_t* deg, const int32_t* round,
+                            int32_t* cand, int r) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t u = list[i];
+    if (round[u] == 0 && deg[u] == 1) cand[u] = r;
+  }
+}
+// Decides removals against start-of-round state: u leaves unless its live
+// neighbour p is also a candidate with p > u (an isolated edge keeps its
+// smaller end).  Removed vertices go to order[base..].
+__global__ void k_peel_decide(const int64_t* off, const int32_t* adj, const int32_t* list, int64_t nl,
+                              const int32_t* round, const int32_t* cand, int r, int32_t* parent,
+                              int32_t* order, u64* ocnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t u = list[i];
+    if (cand[u] != r) continue;
+    int32_t p = -1;
+    for (int64_t e = off[u]; e < off[u + 1]; e++)
+      if (round[adj[e]] == 0) { p = adj[e]; break; }
+    if (p < 0) continue;
+    if (cand[p] == r && p > u) continue;
+    parent[u] = p;
+    order[atomicAdd(ocnt, 1ull)] = u;
+  }
+}
+__global__ void k_peel_apply(const int32_t* rem, int64_t nr, int r, int32_t* round, const int32_t* parent,
+                             u64* w, int32_t* deg, int32_t* next, u64* ncnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nr; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t u = rem[i];
+    int32_t p = parent[u];
+    round[u] = r;
+    atomicAdd(&w[p], w[u] + 1);
+    int old = atomicSub(&deg[p], 1);
+    if (old == 2) next[atomicAdd(ncnt, 1ull)] = p;
+    deg[u] = 0;
+  }
+}
+__global__ void k_peel_root(const int32_t* rem, int64_t nr, const int32_t* round, const int32_t* parent,
+                            int32_t* root) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nr; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t u = rem[i];
+    int32_t p = parent[u];
+    root[u] = round[p] == 0 ? p : root[p];
+  }
+}
+__global__ void k_core_deg(const int64_t* off, const int32_t* adj, int64_t n, const int32_t* round, int64_t* cdeg) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    int64_t c = 0;
+    if (round[v] == 0)
+      for (int64_t e = off[v]; e < off[v + 1]; e++) c += round[adj[e]] == 0;
+    cdeg[v] = c;
+  }
+}
+__global__ void k_core_fill(const int64_t* off, const int32_t* adj, int64_t n, const int32_t* round,
+                            const int64_t* coff, int32_t* cadj) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    if (round[v] != 0) continue;
+    int64_t o = coff[v];
+    for (int64_t e = off[v]; e < off[v + 1]; e++)
+      if (round[adj[e]] == 0) cadj[o++] = adj[e];
+  }
+}
+
+// ===================================================== per-run source terms
+// Multiset counts of the sources, optional expansion to whole pruned trees,
+// and per-component source counts.
+__global__ void k_src_count(const int32_t* src, int64_t ns, u64* cnt0) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ns; i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt0[src[i]], 1ull);
+}
+__global__ void k_src_expand(const int32_t* root, const int32_t* round, int64_t n, const u64* listed, u64* cnt0) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    int32_t rt = root[v];
+    cnt0[v] = (round[rt] == 0 && listed[rt] > 0) ? 1ull : 0ull;
+  }
+}
+__global__ void k_src_init(const u64* cnt0, const int32_t* label, int64_t n, u64* cntS, u64* sa, u64* srcK) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    cntS[v] = cnt0[v];
+    sa[v] = 0;
+    if (cnt0[v]) atomicAdd(&srcK[label[v]], cnt0[v]);
+  }
+}
+// One peel round, leaves first: cntS(p) += cntS(u), sa(p) += cntS(u)*(1+w(u)).
+__global__ void k_tree_acc(const int32_t* rem, int64_t nr, const int32_t* parent, const u64* w, u64* cntS, u64* sa) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nr; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t u = rem[i];
+    int32_t p = parent[u];
+    u64 cs = cntS[u];
+    if (cs) {
+      atomicAdd(&cntS[p], cs);
+      atomicAdd(&sa[p], cs * (1 + w[u]));
+    }
+  }
+}
+// Closed-form dependencies carried by pruned trees (exact-subset semantics):
+// for any vertex x with nK vertices in its component,
+//   bc[x] += desc(x)*(nK-1) - sa(x)                 (sources below x)
+//   bc[x] += w(x)*(|S cap K| - cntS(x))  if x pruned (sources outside x's subtree)
+__global__ void k_finalize(const int32_t* label, const int32_t* round, const u64* w, const u64* cnt0,
+                           const u64* cntS, const u64* sa, const u64* srcK, const int64_t* csize, int64_t n,
+                           double* bc) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    int32_t l = label[v];
+    u64 nK = (u64)csize[l];
+    u64 desc = cntS[v] - cnt0[v];
+    long long t = (long long)(desc * (nK - 1)) - (long long)sa[v];
+    if (round[v] > 0) t += (long long)(w[v] * (srcK[l] - cntS[v]));
+    if (t) bc[v] += (double)t;
+  }
+}
+
+// ============================================================ host side ===
+static int grid_for(int64_t n, int block = 256) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return (int)g;
+}
+
+template <class T>
+static int dalloc(T** p, size_t count, size_t* acc = nullptr) {
+  size_t bytes = sizeof(T) * (count ? count : 1);
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return set_error(BCMI_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  if (acc) *acc += bytes;
+  return BCMI_OK;
+}
+#define TRY(x)               \
+  do {                       \
+    int rc_ = (x);           \
+    if (rc_ != BCMI_OK) return rc_; \
+  } while (0)
+
+void DevGraph::release() {
+  auto fr = [](void* p) { if (p) (void)hipFree(p); };
+  if (toff != off) { fr(toff); fr(tadj); }
+  if (off != foff) { fr(off); fr(adj); }
+  fr(foff); fr(fadj); fr(label); fr(comp_size); fr(comp_arcs); fr(all_v); fr(all_j);
+  fr(round); fr(parent); fr(w); fr(root); fr(order);
+  off = foff = toff = nullptr; adj = fadj = tadj = nullptr; label = nullptr; comp_size = comp_arcs = nullptr;
+  all_v = all_j = nullptr; round = parent = root = order = nullptr; w = nullptr;
+  n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); bytes = 0;
+}
+
+static void chunk_records(const std::vector<int64_t>& off, int chunk, std::vector<int32_t>& rv,
+                          std::vector<int32_t>& rj) {
+  int64_t n = (int64_t)off.size() - 1;
+  rv.clear(); rj.clear();
+  for (int64_t v = 0; v < n; v++) {
+    int64_t dg = off[v + 1] - off[v];
+    if (!dg) continue;
+    int nch = (int)((dg + chunk - 1) / chunk);
+    for (int k = 0; k < nch; k++) { rv.push_back((int32_t)v); rj.push_back(k); }
+  }
+}
+
+int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* adj, int directed,
+                   int heuristic, const Params& prm, hipStream_t st) {
+  g.release();
+  if (n < 1 || n > INT32_MAX) return set_error(BCMI_ERR_ARG, "vertex count out of range");
+  if (off[0] != 0) return set_error(BCMI_ERR_ARG, "offsets must start at 0");
+  for (int64_t v = 0; v < n; v++)
+    if (off[v + 1] < off[v]) return set_error(BCMI_ERR_ARG, "offsets must be non-decreasing");
+  int64_t m = off[n];
+  for (int64_t e = 0; e < m; e++)
+    if (adj[e] < 0 || adj[e] >= n) return set_error(BCMI_ERR_RANGE, "adjacency entry out of range");
+  if (heuristic && directed) return set_error(BCMI_ERR_ARG, "degree-1 peeling needs an undirected graph");
+  g.n = n; g.fm = m; g.directed = directed; g.heuristic = heuristic; g.chunk = prm.chunk;
+  size_t acc = 0;
+  TRY(dalloc(&g.foff, n + 1, &acc));
+  TRY(dalloc(&g.fadj, m, &acc));
+  HIP_TRY(hipMemcpyAsync(g.foff, off, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+  if (m) HIP_TRY(hipMemcpyAsync(g.fadj, adj, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+  TRY(dalloc(&g.label, n, &acc));
+  TRY(dalloc(&g.comp_size, n, &acc));
+  TRY(dalloc(&g.comp_arcs, n, &acc));
+  TRY(dalloc(&g.round, n, &acc));
+  TRY(dalloc(&g.parent, n, &acc));
+  TRY(dalloc(&g.w, n, &acc));
+  TRY(dalloc(&g.root, n, &acc));
+  TRY(dalloc(&g.order, n, &acc));
+  const int B = 256;
+  int G = grid_for(n);
+  hipLaunchKernelGGL(k_fill_i32, G, B, 0, st, g.round, 0, n);
+  hipLaunchKernelGGL(k_fill_i32, G, B, 0, st, g.parent, -1, n);
+  hipLaunchKernelGGL(k_fill_u64, G, B, 0, st, (u64*)g.w, 0ull, n);
+  hipLaunchKernelGGL(k_iota, G, B, 0, st, g.root, n);
+  g.round_lo.assign(1, 0);
+  std::vector<int64_t> hoff(off, off + n + 1);
+  if (heuristic) {
+    int32_t *deg = nullptr, *cand = nullptr, *la = nullptr, *lb = nullptr;
+    u64* cnts = nullptr;  // [0] list a count, [1] next count, [2] order count
+    TRY(dalloc(&deg, n)); TRY(dalloc(&cand, n)); TRY(dalloc(&la, n)); TRY(dalloc(&lb, n));
+    TRY(dalloc(&cnts, 4));
+    u64* hc = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&hc, 4 * sizeof(u64), hipHostMallocDefault));
+    hipLaunchKernelGGL(k_fill_i32, G, B, 0, st, cand, 0, n);
+    hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, st, cnts, 0ull, (int64_t)4);
+    hipLaunchKernelGGL(k_peel_first, G, B, 0, st, g.foff, n, deg, la, cnts);
+    HIP_TRY(hipMemcpyAsync(hc, cnts, 4 * sizeof(u64), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int64_t nl = (int64_t)hc[0];
+    int64_t ocount = 0;
+    int r = 0;
+    while (nl > 0) {
+      r++;
+      int Gl = grid_for(nl);
+      hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, st, cnts + 1, 0ull, (int64_t)1);
+      hipLaunchKernelGGL(k_peel_mark, Gl, B, 0, st, la, nl, deg, g.round, cand, r);
+      hipLaunchKernelGGL(k_peel_decide, Gl, B, 0, st, g.foff, g.fadj, la, nl, g.round, cand, r, g.parent,
+                         g.order, cnts + 2);
+      HIP_TRY(hipMemcpyAsync(hc + 2, cnts + 2, sizeof(u64), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      int64_t nrem = (int64_t)hc[2] - ocount;
+      if (nrem > 0)
+        hipLaunchKernelGGL(k_peel_apply, grid_for(nrem), B, 0, st, g.order + ocount, nrem, r, g.round, g.parent,
+                           (u64*)g.w, deg, lb, cnts + 1);
+      ocount = (int64_t)hc[2];
+      g.round_lo.push_back(ocount);
+      HIP_TRY(hipMemcpyAsync(hc + 1, cnts + 1, sizeof(u64), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      nl = (int64_t)hc[1];
+      std::swap(la, lb);
+      if (nrem == 0 && nl == 0) break;
+    }
+    while (g.round_lo.size() > 1 && g.round_lo[g.round_lo.size() - 1] == g.round_lo[g.round_lo.size() - 2])
+      g.round_lo.pop_back();  // drop trailing rounds that removed nothing
+    g.nrounds = (int32_t)(g.round_lo.size() - 1);
+    // roots, processing rounds from the last one
+    for (int q = g.nrounds; q >= 1; q--) {
+      int64_t a = g.round_lo[q - 1], b = g.round_lo[q];
+      if (b > a) hipLaunchKernelGGL(k_peel_root, grid_for(b - a), B, 0, st, g.order + a, b - a, g.round, g.parent, g.root);
+    }
+    (void)hipFree(deg); (void)hipFree(cand); (void)hipFree(la); (void)hipFree(lb); (void)hipFree(cnts);
+    (void)hipHostFree(hc);
+    if (ocount > 0) {
+      int64_t* cdeg = nullptr;
+      TRY(dalloc(&cdeg, n + 1));
+      hipLaunchKernelGGL(k_core_deg, G, B, 0, st, g.foff, g.fadj, n, g.round, cdeg);
+      std::vector<int64_t> hd(n);
+      HIP_TRY(hipMemcpyAsync(hd.data(), cdeg, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      hoff[0] = 0;
+      for (int64_t v = 0; v < n; v++) hoff[v + 1] = hoff[v] + hd[v];
+      g.m = hoff[n];
+      TRY(dalloc(&g.off, n + 1, &acc));
+      TRY(dalloc(&g.adj, g.m, &acc));
+      HIP_TRY(hipMemcpyAsync(g.off, hoff.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_core_fill, G, B, 0, st, g.foff, g.fadj, n, g.round, g.off, g.adj);
+      HIP_TRY(hipStreamSynchronize(st));
+      (void)hipFree(cdeg);
+    }
+  }
+  if (!g.off) { g.off = g.foff; g.adj = g.fadj; g.m = m; }
+  g.toff = g.off; g.tadj = g.adj;
+  if (directed && m > 0) {
+    // transpose on the host: parents of v are the u with v in adj(u)
+    std::vector<int64_t> to(n + 1, 0);
+    for (int64_t e = 0; e < m; e++) to[adj[e] + 1]++;
+    for (int64_t v = 0; v < n; v++) to[v + 1] += to[v];
+    std::vector<int32_t> ta(m);
+    std::vector<int64_t> cur(to.begin(), to.end() - 1);
+    for (int64_t u = 0; u < n; u++)
+      for (int64_t e = off[u]; e < off[u + 1]; e++) ta[cur[adj[e]]++] = (int32_t)u;
+    g.toff = nullptr; g.tadj = nullptr;
+    TRY(dalloc(&g.toff, n + 1, &acc));
+    TRY(dalloc(&g.tadj, m, &acc));
+    HIP_TRY(hipMemcpyAsync(g.toff, to.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(g.tadj, ta.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  g.h_off = hoff;
+  g.h_round.assign(n, 0);
+  if (heuristic) {
+    HIP_TRY(hipMemcpyAsync(g.h_round.data(), g.round, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  // work records of the traversed graph (bottom-up) and of the uploaded graph (components)
+  std::vector<int32_t> rv, rj;
+  chunk_records(hoff, prm.chunk, rv, rj);
+  g.nall = (int64_t)rv.size();
+  TRY(dalloc(&g.all_v, g.nall, &acc));
+  TRY(dalloc(&g.all_j, g.nall, &acc));
+  if (g.nall) {
+    HIP_TRY(hipMemcpyAsync(g.all_v, rv.data(), sizeof(int32_t) * g.nall, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(g.all_j, rj.data(), sizeof(int32_t) * g.nall, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  {
+    std::vector<int64_t> fo(off, off + n + 1);
+    std::vector<int32_t> fv, fj;
+    chunk_records(fo, prm.chunk, fv, fj);
+    int32_t *dv = nullptr, *dj = nullptr;
+    int64_t nf = (int64_t)fv.size();
+    TRY(dalloc(&dv, nf)); TRY(dalloc(&dj, nf));
+    if (nf) {
+      HIP_TRY(hipMemcpyAsync(dv, fv.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(dj, fj.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
+    }
+    hipLaunchKernelGGL(k_iota, G, B, 0, st, g.label, n);
+    hipLaunchKernelGGL(k_fill_u64, G, B, 0, st, (u64*)g.comp_size, 0ull, n);
+    hipLaunchKernelGGL(k_fill_u64, G, B, 0, st, (u64*)g.comp_arcs, 0ull, n);
+    if (nf) hipLaunchKernelGGL(k_cc_union, 2048, 256, 0, st, g.foff, g.fadj, dv, dj, nf, prm.chunk, g.label);
+    hipLaunchKernelGGL(k_cc_finish, G, B, 0, st, g.label, n, g.off, g.comp_size, g.comp_arcs);
+    HIP_TRY(hipStreamSynchronize(st));
+    (void)hipFree(dv); (void)hipFree(dj);
+  }
+  HIP_TRY(hipGetLastError());
+  g.bytes = acc;
+  return BCMI_OK;
+}
+
+// ------------------------------------------------------------ workspaces
+struct Workspace {
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int mode = 0;
+  int width = 64;
+  int64_t n = 0;
+  double* bc = nullptr;
+  bool own_bc = true;
+  Ctl* ctl = nullptr;
+  Ctl* hctl = nullptr;  // pinned mirror
+  int32_t* rec_v = nullptr;
+  int32_t* rec_j = nullptr;
+  u64* rec_mask = nullptr;
+  int64_t cap = 0;
+  int64_t* rec_lo = nullptr;
+  int32_t* done = nullptr;
+  int32_t* srcbuf = nullptr;
+  double* multbuf = nullptr;
+  int32_t* hsrc = nullptr;   // pinned
+  double* hmult = nullptr;   // pinned
+  // plain
+  int32_t* depth = nullptr;
+  u64* sigma1 = nullptr;
+  double* delta1 = nullptr;
+  double* coef1 = nullptr;
+  // batched
+  u64* seen = nullptr;
+  u64* lm = nullptr;
+  u64* sigma = nullptr;
+  double* coef = nullptr;
+  u64* compmask = nullptr;
+  u64* stats = nullptr;   // 3 x 64
+  u64* hstats = nullptr;  // pinned
+  double* dbg_delta = nullptr;
+  // per-run source terms (peeling)
+  u64* cnt0 = nullptr;
+  u64* cntS = nullptr;
+  u64* sa = nullptr;
+  u64* srcK = nullptr;
+  int32_t* srcdev = nullptr;
+  int64_t srcdev_cap = 0;
+  // captured level loops
+  hipGraphExec_t gfwd = nullptr, gbwd = nullptr;
+  int cap_steps = 0;
+  Params cap_prm;
+  KT* kt = nullptr;             // device timer of the kernel selected by Params::ktime
+  KT* hkt = nullptr;            // pinned mirror
+  const DevGraph* cap_graph = nullptr;
+  size_t bytes = 0;
+};
+
+static GA make_ga(const DevGraph& g) {
+  GA a;
+  a.off = g.off; a.adj = g.adj; a.toff = g.toff; a.tadj = g.tadj; a.directed = g.directed;
+  a.label = g.label; a.comp_arcs = g.comp_arcs;
+  a.all_v = g.all_v; a.all_j = g.all_j; a.nall = g.nall;
+  a.w = g.heuristic ? (const uint64_t*)g.w : nullptr;
+  a.chunk = g.chunk; a.n = g.n;
+  return a;
+}
+static PA make_pa(Workspace* ws, const Params& prm) {
+  PA p;
+  p.ctl = ws->ctl; p.depth = ws->depth; p.sigma = ws->sigma1; p.delta = ws->delta1; p.coef = ws->coef1;
+  p.done = ws->done; p.rec_v = ws->rec_v; p.rec_j = ws->rec_j; p.cap = ws->cap; p.rec_lo = ws->rec_lo;
+  p.srcbuf = ws->srcbuf; p.multbuf = ws->multbuf; p.bc = ws->bc; p.alpha = prm.alpha_plain;
+  return p;
+}
+static BA make_ba(Workspace* ws, const Params& prm) {
+  BA b;
+  b.ctl = ws->ctl; b.seen = ws->seen; b.lm = ws->lm; b.sigma = ws->sigma; b.coef = ws->coef; b.done = ws->done;
+  b.rec_v = ws->rec_v; b.rec_j = ws->rec_j; b.rec_mask = ws->rec_mask; b.cap = ws->cap; b.rec_lo = ws->rec_lo;
+  b.compmask = ws->compmask; b.srcbuf = ws->srcbuf; b.multbuf = ws->multbuf; b.bc = ws->bc;
+  b.st_n = ws->stats; b.st_m = ws->stats + 64; b.st_a = ws->stats + 128; b.dbg_delta = ws->dbg_delta;
+  b.alpha = prm.alpha_batched; b.n = ws->n;
+  return b;
+}
+
+static int64_t record_capacity(const DevGraph& g, int mode) {
+  // Every vertex appears at most once per distinct depth; plain sweeps at
+  // most once.  Start with room for a few depths per vertex; the run grows
+  // the buffer when a sweep reports overflow.
+  int64_t base = g.n + g.m / g.chunk + 1024;
+  return mode == BCMI_MODE_PLAIN ? base : 4 * base;
+}
+
+size_t workspace_bytes(const DevGraph& g, int mode, int width, const Params& prm) {
+  (void)width; (void)prm;
+  int64_t n = g.n;
+  int64_t cap = record_capacity(g, mode);
+  size_t b = sizeof(double) * n + sizeof(int64_t) * (n + 3) + sizeof(int32_t) * n + 2 * sizeof(int32_t) * cap;
+  b += 4 * sizeof(u64) * n;  // per-run source terms
+  if (mode == BCMI_MODE_PLAIN) b += (sizeof(int32_t) + sizeof(u64) + 2 * sizeof(double)) * n;
+  else b += sizeof(u64) * cap + (5 * sizeof(u64) + 64 * (sizeof(u64) + sizeof(double))) * n;
+  return b;
+}
+
+static void free_graphs(Workspace* ws) {
+  if (ws->gfwd) (void)hipGraphExecDestroy(ws->gfwd);
+  if (ws->gbwd) (void)hipGraphExecDestroy(ws->gbwd);
+  ws->gfwd = ws->gbwd = nullptr;
+  ws->cap_graph = nullptr;
+}
+
+__global__ void k_kt_reset(KT* kt) {
+  kt->start = ~0ull;
+  kt->end = 0;
+  kt->total = 0;
+  kt->launches = 0;
+  kt->count = 0;
+}
+
+void workspace_destroy(Workspace* ws) {
+  if (!ws) return;
+  free_graphs(ws);
+  auto fr = [](void* p) { if (p) (void)hipFree(p); };
+  if (ws->own_bc) fr(ws->bc);
+  fr(ws->ctl); fr(ws->rec_v); fr(ws->rec_j); fr(ws->rec_mask); fr(ws->rec_lo); fr(ws->done);
+  fr(ws->srcbuf); fr(ws->multbuf); fr(ws->depth); fr(ws->sigma1); fr(ws->delta1); fr(ws->coef1);
+  fr(ws->seen); fr(ws->lm); fr(ws->sigma); fr(ws->coef); fr(ws->compmask); fr(ws->stats); fr(ws->dbg_delta);
+  fr(ws->cnt0); fr(ws->cntS); fr(ws->sa); fr(ws->srcK); fr(ws->srcdev);
+  if (ws->hctl) (void)hipHostFree(ws->hctl);
+  if (ws->hkt) (void)hipHostFree(ws->hkt);
+  if (ws->kt) (void)hipFree(ws->kt);
+  if (ws->hsrc) (void)hipHostFree(ws->hsrc);
+  if (ws->hmult) (void)hipHostFree(ws->hmult);
+  if (ws->hstats) (void)hipHostFree(ws->hstats);
+  if (ws->ev0) (void)hipEventDestroy(ws->ev0);
+  if (ws->ev1) (void)hipEventDestroy(ws->ev1);
+  if (ws->st) (void)hipStreamDestroy(ws->st);
+  delete ws;
+}
+
+static int alloc_records(Workspace* ws, int64_t cap) {
+  auto fr = [](void* p) { if (p) (void)hipFree(p); };
+  fr(ws->rec_v); fr(ws->rec_j); fr(ws->rec_mask);
+  ws->rec_v = nullptr; ws->rec_j = nullptr; ws->rec_mask = nullptr;
+  ws->cap = cap;
+  TRY(dalloc(&ws->rec_v, cap, &ws->bytes));
+  TRY(dalloc(&ws->rec_j, cap, &ws->bytes));
+  if (ws->mode == BCMI_MODE_BATCHED) TRY(dalloc(&ws->rec_mask, cap, &ws->bytes));
+  free_graphs(ws);
+  return BCMI_OK;
+}
+
+static int ws_init(Workspace* ws, const DevGraph& g, int mode, int width, const Params& prm) {
+  int64_t n = g.n;
+  ws->mode = mode; ws->width = width; ws->n = n;
+  HIP_TRY(hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&ws->ev0));
+  HIP_TRY(hipEventCreate(&ws->ev1));
+  size_t* acc = &ws->bytes;
+  TRY(dalloc(&ws->bc, n, acc));
+  TRY(dalloc(&ws->ctl, 1, acc));
+  HIP_TRY(hipHostMalloc((void**)&ws->hctl, sizeof(Ctl), hipHostMallocDefault));
+  TRY(dalloc(&ws->kt, 1, acc));
+  HIP_TRY(hipHostMalloc((void**)&ws->hkt, sizeof(KT), hipHostMallocDefault));
+  TRY(dalloc(&ws->rec_lo, n + 3, acc));
+  TRY(dalloc(&ws->done, n, acc));
+  TRY(dalloc(&ws->srcbuf, 64, acc));
+  TRY(dalloc(&ws->multbuf, 64, acc));
+  HIP_TRY(hipHostMalloc((void**)&ws->hsrc, 64 * sizeof(int32_t), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&ws->hmult, 64 * sizeof(double), hipHostMallocDefault));
+  TRY(dalloc(&ws->cnt0, n, acc));
+  TRY(dalloc(&ws->cntS, n, acc));
+  TRY(dalloc(&ws->sa, n, acc));
+  TRY(dalloc(&ws->srcK, n, acc));
+  int G = grid_for(n);
+  hipStream_t st = ws->st;
+  hipLaunchKernelGGL(k_fill_f64, G, 256, 0, st, ws->bc, 0.0, n);
+  hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, ws->done, 0, n);
+  hipLaunchKernelGGL(k_kt_reset, 1, 1, 0, st, ws->kt);
+  hipLaunchKernelGGL(k_fill_u64, grid_for(n + 3), 256, 0, st, (u64*)ws->rec_lo, 0ull, n + 3);
+  if (mode == BCMI_MODE_PLAIN) {
+    TRY(dalloc(&ws->depth, n, acc));
+    TRY(dalloc(&ws->sigma1, n, acc));
+    TRY(dalloc(&ws->delta1, n, acc));
+    TRY(dalloc(&ws->coef1, n, acc));
+    hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, ws->depth, -1, n);
+    hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->sigma1, 0ull, n);
+    hipLaunchKernelGGL(k_fill_f64, G, 256, 0, st, ws->delta1, 0.0, n);
+    hipLaunchKernelGGL(k_fill_f64, G, 256, 0, st, ws->coef1, 0.0, n);
+  } else {
+    TRY(dalloc(&ws->seen, n, acc));
+    TRY(dalloc(&ws->lm, 3 * n, acc));
+    TRY(dalloc(&ws->sigma, 64 * n, acc));
+    TRY(dalloc(&ws->coef, 64 * n, acc));
+    TRY(dalloc(&ws->compmask, n, acc));
+    TRY(dalloc(&ws->stats, 3 * 64, acc));
+    HIP_TRY(hipHostMalloc((void**)&ws->hstats, 3 * 64 * sizeof(u64), hipHostMallocDefault));
+    hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->seen, 0ull, n);
+    hipLaunchKernelGGL(k_fill_u64, grid_for(3 * n), 256, 0, st, ws->lm, 0ull, 3 * n);
+    hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->compmask, 0ull, n);
+  }
+  TRY(alloc_records(ws, record_capacity(g, mode)));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipGetLastError());
+  return BCMI_OK;
+}
+
+Workspace* workspace_create(const DevGraph& g, int mode, int width, const Params& prm, int* rc) {
+  Workspace* ws = new Workspace();
+  *rc = ws_init(ws, g, mode, width, prm);
+  if (*rc != BCMI_OK) {
+    workspace_destroy(ws);
+    return nullptr;
+  }
+  return ws;
+}
+hipStream_t workspace_stream(Workspace* ws) { return ws->st; }
+size_t workspace_size(Workspace* ws) { return ws->bytes; }
+void workspace_share_bc(Workspace* ws, double* bc) {
+  if (ws->own_bc && ws->bc) (void)hipFree(ws->bc);
+  ws->bc = bc;
+  ws->own_bc = false;
+  free_graphs(ws);
+}
+double* workspace_bc(Workspace* ws) { return ws->bc; }
+int workspace_bc_reset(Workspace* ws) {
+  hipLaunchKernelGGL(k_fill_f64, grid_for(ws->n), 256, 0, ws->st, ws->bc, 0.0, ws->n);
+  HIP_TRY(hipStreamSynchronize(ws->st));
+  return BCMI_OK;
+}
+
+// Captures `steps` forward levels and `steps` backward levels into two
+// graphs (kernel nodes only).
+static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
+  free_graphs(ws);
+  GA ga = make_ga(g);
+  hipStream_t st = ws->st;
+  int B = prm.block, G = prm.grid;
+  // kernel timing: which kernel gets the device timer
+  const int kt = prm.ktime;
+  const bool plain = ws->mode == BCMI_MODE_PLAIN;
+  KT* T = ws->kt;
+  auto tk = [&](int id) -> KT* { return kt == id ? T : nullptr; };
+  for (int which = 0; which < 2; which++) {
+    hipGraph_t graph;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < prm.steps; s++) {
+      if (plain) {
+        PA pa = make_pa(ws, prm);
+        if (which == 0) {
+          hipLaunchKernelGGL(k_plain_decide, 1, 1, 0, st, ga, pa);
+          hipLaunchKernelGGL(k_plain_td, G, B, 0, st, ga, pa, tk(5));
+          hipLaunchKernelGGL(k_plain_bu, G, B, 0, st, ga, pa, tk(6));
+        } else {
+          hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, pa.ctl);
+          hipLaunchKernelGGL(k_plain_back, G, B, 0, st, ga, pa, tk(7));
+        }
+      } else {
+        BA ba = make_ba(ws, prm);
+        if (which == 0) {
+          hipLaunchKernelGGL(k_ms_decide, 1, 1, 0, st, ga, ba);
+          hipLaunchKernelGGL(k_ms_td, G, B, 0, st, ga, ba, tk(3));
+          hipLaunchKernelGGL(k_ms_bu, G, B, 0, st, ga, ba, tk(4));
+          hipLaunchKernelGGL(k_ms_sigma, G, B, 0, st, ga, ba, tk(1));
+        } else {
+          hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, ba.ctl);
+          hipLaunchKernelGGL(k_ms_back, G, B, 0, st, ga, ba, tk(2));
+        }
+      }
+    }
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) return set_error(BCMI_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex;
+    e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return set_error(BCMI_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    if (which == 0) ws->gfwd = ex; else ws->gbwd = ex;
+  }
+  ws->cap_steps = prm.steps;
+  ws->cap_prm = prm;
+  ws->cap_graph = &g;
+  return BCMI_OK;
+}
+
+static bool same_launch(const Params& a, const Params& b) {
+  return a.block == b.block && a.grid == b.grid && a.chunk == b.chunk && a.alpha_plain == b.alpha_plain &&
+         a.alpha_batched == b.alpha_batched && a.steps == b.steps && a.ktime == b.ktime;
+}
+
+// Runs one sweep (plain: srcbuf[0]; batched: srcbuf[0..k)) whose inputs are
+// already in the workspace buffers.  keep != 0 skips the cleanup (debug
+// readback), the caller then calls sweep_cleanup.
+static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, bool keep, int* levels) {
+  if (!ws->gfwd || ws->cap_graph != &g || !same_launch(ws->cap_prm, prm)) TRY(capture(g, ws, prm));
+  GA ga = make_ga(g);
+  hipStream_t st = ws->st;
+  for (int attempt = 0;; attempt++) {
+    if (ws->mode == BCMI_MODE_PLAIN) {
+      PA pa = make_pa(ws, prm);
+      hipLaunchKernelGGL(k_plain_init, 1, 1, 0, st, ga, pa);
+    } else {
+      BA ba = make_ba(ws, prm);
+      hipLaunchKernelGGL(k_ms_init, 1, 64, 0, st, ga, ba, k);
+    }
+    HIP_TRY(hipGraphLaunch(ws->gfwd, st));
+    HIP_TRY(hipGraphLaunch(ws->gbwd, st));
+    HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    while (!ws->hctl->fdone && !ws->hctl->overflow) {
+      HIP_TRY(hipGraphLaunch(ws->gfwd, st));
+      HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    if (ws->hctl->overflow) {
+      // Work list too small: reset every state array and retry with twice the room.
+      int64_t n = ws->n;
+      int G = grid_for(n);
+      if (ws->mode == BCMI_MODE_PLAIN) {
+        hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, ws->depth, -1, n);
+        hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->sigma1, 0ull, n);
+      } else {
+        hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->seen, 0ull, n);
+        hipLaunchKernelGGL(k_fill_u64, grid_for(3 * n), 256, 0, st, ws->lm, 0ull, 3 * n);
+        hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->compmask, 0ull, n);
+      }
+      hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, ws->done, 0, n);
+      HIP_TRY(hipStreamSynchronize(st));
+      if (attempt > 8) return set_error(BCMI_ERR_NOMEM, "work list keeps overflowing");
+      TRY(alloc_records(ws, ws->cap * 2));
+      TRY(capture(g, ws, prm));
+      continue;
+    }
+    while (!ws->hctl->bdone) {
+      HIP_TRY(hipGraphLaunch(ws->gbwd, st));
+      HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    break;
+  }
+  if (levels) *levels = ws->hctl->last + 1;
+  if (!keep) {
+    if (ws->mode == BCMI_MODE_PLAIN) {
+      PA pa = make_pa(ws, prm);
+      hipLaunchKernelGGL(k_plain_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, st, pa);
+    } else {
+      BA ba = make_ba(ws, prm);
+      hipLaunchKernelGGL(k_ms_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, st, ga, ba);
+    }
+  }
+  return BCMI_OK;
+}
+
+static int sweep_cleanup(const DevGraph& g, Workspace* ws, const Params& prm) {
+  GA ga = make_ga(g);
+  if (ws->mode == BCMI_MODE_PLAIN) {
+    PA pa = make_pa(ws, prm);
+    hipLaunchKernelGGL(k_plain_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, ws->st, pa);
+  } else {
+    BA ba = make_ba(ws, prm);
+    hipLaunchKernelGGL(k_ms_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, ws->st, ga, ba);
+  }
+  HIP_TRY(hipStreamSynchronize(ws->st));
+  return BCMI_OK;
+}
+
+// Source preparation.  Without peeling every listed source of degree > 0 is
+// swept with multiplicity 1.  With peeling (or expansion) sources are first
+// pushed down to the core roots of their trees.
+static int prepare_sources(const DevGraph& g, Workspace* ws, const int32_t* src, int64_t nsrc, int src_flags,
+                           std::vector<int32_t>& cs, std::vector<double>& cm, bool& finalize) {
+  cs.clear(); cm.clear();
+  finalize = false;
+  const std::vector<int64_t>& ho = g.h_off;
+  for (int64_t i = 0; i < nsrc; i++)
+    if (src[i] < 0 || src[i] >= g.n) return set_error(BCMI_ERR_RANGE, "source id out of range");
+  if (!g.heuristic) {
+    for (int64_t i = 0; i < nsrc; i++)
+      if (ho[src[i] + 1] > ho[src[i]]) { cs.push_back(src[i]); cm.push_back(1.0); }
+    return BCMI_OK;
+  }
+  finalize = true;
+  hipStream_t st = ws->st;
+  int64_t n = g.n;
+  if (nsrc > ws->srcdev_cap) {
+    if (ws->srcdev) (void)hipFree(ws->srcdev);
+    ws->srcdev = nullptr;
+    TRY(dalloc(&ws->srcdev, nsrc));
+    ws->srcdev_cap = nsrc;
+  }
+  if (nsrc) HIP_TRY(hipMemcpyAsync(ws->srcdev, src, sizeof(int32_t) * nsrc, hipMemcpyHostToDevice, st));
+  int G = grid_for(n);
+  hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->cnt0, 0ull, n);
+  hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->srcK, 0ull, n);
+  if (nsrc) hipLaunchKernelGGL(k_src_count, grid_for(nsrc), 256, 0, st, ws->srcdev, nsrc, ws->cnt0);
+  if (src_flags & BCMI_SRC_EXPAND) {
+    hipLaunchKernelGGL(k_src_expand, G, 256, 0, st, g.root, g.round, n, ws->cnt0, ws->cntS);
+    HIP_TRY(hipMemcpyAsync(ws->cnt0, ws->cntS, sizeof(u64) * n, hipMemcpyDeviceToDevice, st));
+  }
+  hipLaunchKernelGGL(k_src_init, G, 256, 0, st, ws->cnt0, g.label, n, ws->cntS, ws->sa, ws->srcK);
+  for (int q = 1; q <= g.nrounds; q++) {
+    int64_t a = g.round_lo[q - 1], b = g.round_lo[q];
+    if (b > a) hipLaunchKernelGGL(k_tree_acc, grid_for(b - a), 256, 0, st, g.order + a, b - a, g.parent, (const u64*)g.w, ws->cntS, ws->sa);
+  }
+  std::vector<u64> hc(n);
+  HIP_TRY(hipMemcpyAsync(hc.data(), ws->cntS, sizeof(u64) * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  // core roots in increasing id order; pruned vertices have no traversed arcs
+  const std::vector<int32_t>& rnd = g.h_round;
+  for (int64_t v = 0; v < n; v++)
+    if (rnd[v] == 0 && hc[v] > 0 && ho[v + 1] > ho[v]) { cs.push_back((int32_t)v); cm.push_back((double)hc[v]); }
+  return BCMI_OK;
+}
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* sources, int64_t nsrc, int mode,
+           int src_flags, RunResult* out) {
+  if (mode != ws->mode) return set_error(BCMI_ERR_STATE, "workspace was created for another mode");
+  if ((src_flags & BCMI_SRC_EXPAND) && !g.heuristic)
+    return set_error(BCMI_ERR_ARG, "source expansion needs a peeled graph");
+  double t0 = now_ms();
+  std::vector<int32_t> cs;
+  std::vector<double> cm;
+  bool fin = false;
+  TRY(prepare_sources(g, ws, sources, nsrc, src_flags, cs, cm, fin));
+  RunResult R;
+  hipStream_t st = ws->st;
+  hipLaunchKernelGGL(k_kt_reset, 1, 1, 0, st, ws->kt);
+  HIP_TRY(hipEventRecord(ws->ev0, st));
+  int ovf = 0;
+  if (mode == BCMI_MODE_PLAIN) {
+    for (size_t i = 0; i < cs.size(); i++) {
+      ws->hsrc[0] = cs[i];
+      ws->hmult[0] = cm[i];
+      HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, sizeof(int32_t), hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, sizeof(double), hipMemcpyHostToDevice, st));
+      int lv = 0;
+      TRY(sweep(g, ws, prm, 1, false, &lv));
+      const Ctl& c = *ws->hctl;
+      R.n_s.push_back(c.tot_verts);
+      R.m_s.push_back(c.tot_arcs);
+      R.a_s.push_back(c.sum_a);
+      R.n_td += c.n_td; R.n_bu += c.n_bu;
+      R.sweeps++; R.launches++;
+      R.max_levels = std::max<int64_t>(R.max_levels, lv);
+      R.levels.push_back(lv);
+      if (c.sigma_ovf) ovf = 1;
+    }
+  } else {
+    int W = std::max(1, std::min(64, prm.width));
+    for (size_t i = 0; i < cs.size(); i += W) {
+      int k = (int)std::min<size_t>(W, cs.size() - i);
+      for (int q = 0; q < 64; q++) {
+        ws->hsrc[q] = q < k ? cs[i + q] : 0;
+        ws->hmult[q] = q < k ? cm[i + q] : 0.0;
+      }
+      HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, 64 * sizeof(int32_t), hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, 64 * sizeof(double), hipMemcpyHostToDevice, st));
+      int lv = 0;
+      TRY(sweep(g, ws, prm, k, false, &lv));
+      HIP_TRY(hipMemcpyAsync(ws->hstats, ws->stats, 3 * 64 * sizeof(u64), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      for (int q = 0; q < k; q++) {
+        R.n_s.push_back((int64_t)ws->hstats[q]);
+        R.m_s.push_back((int64_t)ws->hstats[64 + q]);
+        R.a_s.push_back((int64_t)ws->hstats[128 + q]);
+        R.levels.push_back(lv);
+      }
+      if (ws->hctl->sigma_ovf) ovf = 1;
+      R.uni_n.push_back(ws->hctl->uni_verts);
+      R.uni_m.push_back(ws->hctl->uni_arcs);
+      R.n_td += ws->hctl->n_td; R.n_bu += ws->hctl->n_bu;
+      R.sweeps += k;
+      R.launches++;
+      R.max_levels = std::max<int64_t>(R.max_levels, lv);
+    }
+  }
+  if (fin) {
+    hipLaunchKernelGGL(k_finalize, grid_for(g.n), 256, 0, st, g.label, g.round, (const u64*)g.w, ws->cnt0, ws->cntS,
+                       ws->sa, ws->srcK, g.comp_size, g.n, ws->bc);
+  }
+  HIP_TRY(hipEventRecord(ws->ev1, st));
+  HIP_TRY(hipEventSynchronize(ws->ev1));
+  HIP_TRY(hipGetLastError());
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
+  R.gpu_ms = ms;
+  R.wall_ms = now_ms() - t0;
+  HIP_TRY(hipMemcpy(ws->hkt, ws->kt, sizeof(KT), hipMemcpyDeviceToHost));
+  {
+    int dev = 0, khz = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    R.ktime_ms = khz > 0 ? (double)ws->hkt->total / (double)khz : 0.0;
+    R.klaunches = (int64_t)ws->hkt->launches;
+  }
+  for (size_t i = 0; i < R.n_s.size(); i++) {
+    R.sum_n += R.n_s[i]; R.sum_m += R.m_s[i]; R.sum_a += R.a_s[i];
+  }
+  R.te = 2.0 * (double)R.sum_m;
+  // Algorithmic HBM bytes (frozen model):
+  //   plain   B(s) = 16 m_s + 24 a_s + 64 n_s
+  //   batched B(S) = 24 m_S + 32 n_S + sum_s (24 a_s + 32 n_s), m_S/n_S = union reach
+  R.bytes = 0;
+  if (mode == BCMI_MODE_PLAIN) {
+    for (size_t i = 0; i < R.n_s.size(); i++)
+      R.bytes += 16.0 * R.m_s[i] + 24.0 * R.a_s[i] + 64.0 * R.n_s[i];
+  } else {
+    for (size_t i = 0; i < R.uni_n.size(); i++) R.bytes += 24.0 * R.uni_m[i] + 32.0 * R.uni_n[i];
+    for (size_t i = 0; i < R.n_s.size(); i++) R.bytes += 24.0 * R.a_s[i] + 32.0 * R.n_s[i];
+  }
+  if (out) *out = R;
+  if (ovf) return set_error(BCMI_ERR_OVERFLOW, "a shortest-path count exceeded 2^64-1");
+  return BCMI_OK;
+}
+
+int sssp_debug(const DevGraph& g, Workspace* ws, const Params& prm, int32_t s, int32_t* depth, uint64_t* sigma,
+               double* delta) {
+  if (ws->mode != BCMI_MODE_PLAIN) return set_error(BCMI_ERR_STATE, "plain workspace required");
+  if (s < 0 || s >= g.n) return set_error(BCMI_ERR_RANGE, "source out of range");
+  hipStream_t st = ws->st;
+  // Sweep into a scratch BC (bc unchanged): the multiplier is 0.
+  ws->hsrc[0] = s;
+  ws->hmult[0] = 0.0;
+  HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, sizeof(double), hipMemcpyHostToDevice, st));
+  int lv;
+  TRY(sweep(g, ws, prm, 1, true, &lv));
+  int64_t n = g.n;
+  HIP_TRY(hipMemcpyAsync(depth, ws->depth, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(sigma, ws->sigma1, sizeof(u64) * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(delta, ws->delta1, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int64_t v = 0; v < n; v++)
+    if (depth[v] < 0) { sigma[v] = 0; delta[v] = 0.0; }
+  // the source's delta is not computed by the sweep (depths D-2..1 only)
+  delta[s] = 0.0;
+  int rc = sweep_cleanup(g, ws, prm);
+  if (ws->hctl->sigma_ovf) return set_error(BCMI_ERR_OVERFLOW, "a shortest-path count exceeded 2^64-1");
+  return rc;
+}
+
+int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* src, int k, int32_t* depth,
+                uint64_t* sigma, double* delta) {
+  if (ws->mode != BCMI_MODE_BATCHED) return set_error(BCMI_ERR_STATE, "batched workspace required");
+  if (k < 1 || k > 64) return set_error(BCMI_ERR_ARG, "1..64 sources");
+  for (int q = 0; q < k; q++)
+    if (src[q] < 0 || src[q] >= g.n) return set_error(BCMI_ERR_RANGE, "source out of range");
+  hipStream_t st = ws->st;
+  int64_t n = g.n;
+  if (!ws->dbg_delta) {
+    TRY(dalloc(&ws->dbg_delta, 64 * n));
+    free_graphs(ws);
+  }
+  hipLaunchKernelGGL(k_fill_f64, grid_for(64 * n), 256, 0, st, ws->dbg_delta, 0.0, 64 * n);
+  for (int q = 0; q < 64; q++) {
+    ws->hsrc[q] = q < k ? src[q] : 0;
+    ws->hmult[q] = 0.0;
+  }
+  HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, 64 * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, 64 * sizeof(double), hipMemcpyHostToDevice, st));
+  int lv;
+  int rc = sweep(g, ws, prm, k, true, &lv);
+  if (rc != BCMI_OK) return rc;
+  int64_t nrec = std::min<int64_t>(ws->hctl->rec_count, ws->cap);
+  int last = ws->hctl->last;
+  std::vector<int32_t> rv(nrec), rj(nrec);
+  std::vector<u64> rm(nrec);
+  std::vector<int64_t> lo(last + 2);
+  std::vector<u64> sg((size_t)64 * n);
+  std::vector<double> dd((size_t)64 * n);
+  HIP_TRY(hipMemcpyAsync(rv.data(), ws->rec_v, sizeof(int32_t) * nrec, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(rj.data(), ws->rec_j, sizeof(int32_t) * nrec, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(rm.data(), ws->rec_mask, sizeof(u64) * nrec, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(lo.data(), ws->rec_lo, sizeof(int64_t) * (last + 2), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(sg.data(), ws->sigma, sizeof(u64) * 64 * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(dd.data(), ws->dbg_delta, sizeof(double) * 64 * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int q = 0; q < k; q++)
+    for (int64_t v = 0; v < n; v++) { depth[(int64_t)q * n + v] = -1; sigma[(int64_t)q * n + v] = 0; delta[(int64_t)q * n + v] = 0.0; }
+  for (int d = 0; d <= last; d++)
+    for (int64_t r = lo[d]; r < lo[d + 1]; r++) {
+      if (rj[r] != 0) continue;
+      int32_t v = rv[r];
+      u64 m = rm[r];
+      for (int q = 0; q < k; q++)
+        if ((m >> q) & 1) {
+          depth[(int64_t)q * n + v] = d;
+          sigma[(int64_t)q * n + v] = sg[(size_t)v * 64 + q];
+          delta[(int64_t)q * n + v] = d >= 1 ? dd[(size_t)v * 64 + q] : 0.0;
+        }
+    }
+  rc = sweep_cleanup(g, ws, prm);
+  if (ws->hctl->sigma_ovf) return set_error(BCMI_ERR_OVERFLOW, "a shortest-path count exceeded 2^64-1");
+  return rc;
+}
+
+}  // namespace bcmi

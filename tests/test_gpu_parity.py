@@ -1,0 +1,221 @@
+"""GPU parity: the HIP engine against the CPU oracle on seeded graphs.
+
+Depths and sigma must match bit for bit; delta and BC within 1e-9 relative
+(the bound the engine promises for FP64 accumulation)."""
+import numpy as np
+import pytest
+
+import bcmi
+from oracle import oracle as O
+import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+BC_TOL = 1e-9
+
+
+def csr(edges, n, directed=False):
+    return bcmi.Graph.from_edges(edges, n, directed=directed)
+
+
+CASES = {
+    "path9": G.path(9),
+    "star7": G.star(7),
+    "cycle10": G.cycle(10),
+    "k6": G.complete(6),
+    "grid5x7": G.grid(5, 7),
+    "cube6": G.hypercube(6),
+    "rand60": G.random_graph(60, 150, 3),
+    "rand200": G.random_graph(200, 500, 4),
+    "forest": G.random_forest_plus(150, 5),
+}
+
+
+def _set_dir(engine, mode):
+    # mode: "auto", "td" (never bottom-up), "bu" (always bottom-up)
+    a = {"auto": 1000, "td": 1 << 60, "bu": 0}[mode]
+    engine.set_param("alpha_plain", a)
+    engine.set_param("alpha_batched", a)
+
+
+@pytest.fixture(autouse=True)
+def _restore(engine):
+    yield
+    _set_dir(engine, "auto")
+
+
+@pytest.mark.parametrize("direction", ["td", "bu", "auto"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_plain_sssp_exact(engine, name, direction):
+    e, n = CASES[name]
+    g = csr(e, n)
+    engine.upload(g)
+    _set_dir(engine, direction)
+    for s in range(0, n, max(1, n // 7)):
+        d, sg, dl = engine.sssp(s)
+        od, osg, odl, _ = O.sssp(g.off, g.adj, s)
+        assert np.array_equal(d, od), (name, s)
+        assert np.array_equal(sg, osg), (name, s)
+        odl[s] = 0.0
+        assert bcmi.rel_err(dl, odl) <= 1e-12, (name, s)
+
+
+@pytest.mark.parametrize("direction", ["td", "bu", "auto"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_batched_msbfs_exact(engine, name, direction):
+    e, n = CASES[name]
+    g = csr(e, n)
+    engine.upload(g)
+    _set_dir(engine, direction)
+    src = np.arange(0, n, max(1, n // 64))[:64].astype(np.int32)
+    d, sg, dl = engine.msbfs(src)
+    for q, s in enumerate(src):
+        od, osg, odl, _ = O.sssp(g.off, g.adj, int(s))
+        assert np.array_equal(d[q], od), (name, s)
+        assert np.array_equal(sg[q], osg), (name, s)
+        odl[s] = 0.0
+        # lanes sum successors in adjacency order like the oracle
+        assert bcmi.rel_err(dl[q], odl) <= 1e-12, (name, s)
+
+
+@pytest.mark.parametrize("mode", [bcmi.MODE_PLAIN, bcmi.MODE_BATCHED])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_bc_all_sources(engine, name, mode):
+    e, n = CASES[name]
+    g = csr(e, n)
+    engine.upload(g)
+    engine.reset()
+    engine.run(np.arange(n), mode=mode)
+    bc = engine.bc()
+    ref = O.brandes(g.off, g.adj, np.arange(n))
+    assert bcmi.rel_err(bc, ref) <= BC_TOL, name
+
+
+@pytest.mark.parametrize("mode", [bcmi.MODE_PLAIN, bcmi.MODE_BATCHED])
+def test_rmat_scale10_config(engine, mode):
+    """R-MAT scale 10, edge factor 16, all 1,024 sources (isolated skipped)."""
+    g = bcmi.Graph.rmat(10, 16, seed=20240101)
+    engine.upload(g)
+    engine.reset()
+    st = engine.run(np.arange(g.n), mode=mode)
+    bc = engine.bc()
+    ref, ost = O.brandes(g.off, g.adj, np.arange(g.n), nthreads=4, stats=True)
+    assert bcmi.rel_err(bc, ref) <= BC_TOL
+    deg = g.degrees()
+    live = deg[np.arange(g.n)] > 0
+    assert st["sources_run"] == int(live.sum())
+    assert st["sum_m"] == int(ost["m_s"][live].sum())
+    assert st["sum_a"] == int(ost["a_s"][live].sum())
+    assert st["sum_n"] == int(ost["n_s"][live].sum())
+
+
+def test_directed_in_arcs(engine):
+    """-U: adjacency holds in-neighbours (CSC); BFS walks them like the reference."""
+    e, n = G.random_graph(120, 400, 11)
+    g = csr(e, n, directed=True)
+    engine.upload(g)
+    for mode in (bcmi.MODE_PLAIN, bcmi.MODE_BATCHED):
+        engine.reset()
+        engine.run(np.arange(n), mode=mode)
+        ref = O.brandes(g.off, g.adj, np.arange(n))
+        assert bcmi.rel_err(engine.bc(), ref) <= BC_TOL
+
+
+def test_sigma_exact_near_2_64(engine):
+    e, n = G.diamonds(63)
+    g = csr(e, n)
+    engine.upload(g)
+    d, sg, _ = engine.sssp(0)
+    assert int(sg[n - 1]) == 1 << 63
+    dm, sgm, _ = engine.msbfs(np.array([0], np.int32))
+    assert int(sgm[0, n - 1]) == 1 << 63
+
+
+@pytest.mark.parametrize("mode", [bcmi.MODE_PLAIN, bcmi.MODE_BATCHED])
+def test_sigma_overflow_reported(engine, mode):
+    e, n = G.diamonds(64)
+    g = csr(e, n)
+    engine.upload(g)
+    engine.reset()
+    with pytest.raises(bcmi.BcmiError) as ex:
+        engine.run(np.array([0], np.int32), mode=mode)
+    assert ex.value.code == bcmi.ERR_OVERFLOW
+    # the engine stays usable
+    engine.reset()
+    engine.run(np.array([1], np.int32), mode=mode)
+
+
+def test_duplicate_sources_count_twice(engine):
+    e, n = G.random_graph(80, 200, 9)
+    g = csr(e, n)
+    engine.upload(g)
+    src = np.array([3, 3, 5, 7, 7, 7], np.int32)
+    ref = O.brandes(g.off, g.adj, src)
+    for mode in (bcmi.MODE_PLAIN, bcmi.MODE_BATCHED):
+        engine.reset()
+        engine.run(src, mode=mode)
+        assert bcmi.rel_err(engine.bc(), ref) <= BC_TOL
+
+
+def test_batch_width_and_many_batches(engine):
+    g = bcmi.Graph.rmat(11, 8, seed=5)
+    engine.upload(g)
+    src = g.sample_sources(300, seed=2)
+    ref = O.brandes(g.off, g.adj, src, nthreads=4)
+    for width in (64, 17, 1):
+        engine.set_param("width", width)
+        engine.reset()
+        engine.run(src, mode=bcmi.MODE_BATCHED)
+        assert bcmi.rel_err(engine.bc(), ref) <= BC_TOL, width
+    engine.set_param("width", 64)
+
+
+def test_small_chunks_split_hubs(engine):
+    """Chunk 64 forces multi-record vertices (atomic partial sums)."""
+    e2 = bcmi.Engine(chunk=64)
+    g = bcmi.Graph.rmat(11, 16, seed=8)
+    e2.upload(g)
+    src = g.sample_sources(128, seed=4)
+    ref = O.brandes(g.off, g.adj, src, nthreads=4)
+    for mode in (bcmi.MODE_PLAIN, bcmi.MODE_BATCHED):
+        e2.reset()
+        e2.run(src, mode=mode)
+        assert bcmi.rel_err(e2.bc(), ref) <= BC_TOL
+    d, sg, dl = e2.msbfs(src[:64])
+    for q in range(0, 64, 9):
+        od, osg, odl, _ = O.sssp(g.off, g.adj, int(src[q]))
+        assert np.array_equal(d[q], od) and np.array_equal(sg[q], osg)
+    e2.close()
+
+
+def test_deep_chain_many_graph_replays(engine):
+    """Diameter far above the levels captured per graph (device-driven loop)."""
+    g = bcmi.Graph.block_chain(40, 64, 6.0, seed=3)
+    engine.upload(g)
+    src = np.array([0, 5, g.n // 2, g.n - 1], np.int32)
+    ref = O.brandes(g.off, g.adj, src)
+    for mode in (bcmi.MODE_PLAIN, bcmi.MODE_BATCHED):
+        engine.reset()
+        st = engine.run(src, mode=mode)
+        assert st["max_levels"] > 16
+        assert bcmi.rel_err(engine.bc(), ref) <= BC_TOL
+
+
+def test_kernel_timer(engine):
+    g = bcmi.Graph.rmat(12, 16, seed=3)
+    engine.upload(g)
+    src = g.sample_sources(128, seed=1)
+    ref = O.brandes(g.off, g.adj, src, nthreads=4)
+    for kt in (1, 2, 3):
+        engine.set_param("ktime", kt)
+        engine.reset()
+        st = engine.run(src, mode=bcmi.MODE_BATCHED)
+        ms, launches = engine.kernel_time()
+        assert launches >= st["batches"] * 2 and 0 < ms < st["wall_ms"], (kt, ms, launches)
+        assert bcmi.rel_err(engine.bc(), ref) <= BC_TOL
+    engine.set_param("ktime", 7)
+    engine.reset()
+    engine.run(src[:8], mode=bcmi.MODE_PLAIN)
+    ms, launches = engine.kernel_time()
+    assert launches > 0 and ms > 0
+    engine.set_param("ktime", 0)
