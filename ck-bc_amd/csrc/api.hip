@@ -115,6 +115,8 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "width") {
     if (v < 1 || v > 64) return set_error(BCMI_ERR_ARG, "width must be in [1,64]");
     p.width = (int)v;
+  } else if (k == "trace") {
+    p.trace = v ? 1 : 0;
   } else if (k == "ktime") {
     if (v < 0 || v > 7) return set_error(BCMI_ERR_ARG, "ktime must be in [0,7]");
     p.ktime = (int)v;
@@ -137,6 +139,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "steps") *v = p.steps;
   else if (k == "width") *v = p.width;
   else if (k == "ktime") *v = p.ktime;
+  else if (k == "trace") *v = p.trace;
   else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
   return BCMI_OK;
 }

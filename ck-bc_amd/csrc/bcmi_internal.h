@@ -24,6 +24,7 @@ struct Params {
   int64_t alpha_batched = 1000; // x1000
   int steps = 16;           // levels per captured graph
   int width = 64;           // sources per batch
+  int trace = 0;            // diagnostic: per-level statistics on stderr (use with steps = 1)
   int ktime = 0;            // kernel timed with HIP events inside the level loop:
                             // 0 none, 1 batched sigma, 2 batched backward, 3 batched top-down,
                             // 4 batched bottom-up, 5 plain top-down, 6 plain bottom-up, 7 plain backward

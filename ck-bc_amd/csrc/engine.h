@@ -69,6 +69,7 @@ struct DevGraph {
   std::vector<int64_t> round_lo; // offsets of each round in order (size nrounds+1)
   std::vector<int64_t> h_off;    // traversed offsets on the host (degrees)
   std::vector<int32_t> h_round;  // removal round per vertex on the host
+  std::vector<uint32_t> h_sig;   // batching key: BFS depths from the four largest hubs
   size_t bytes = 0;
   void release();
 };

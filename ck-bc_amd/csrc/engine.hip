@@ -39,6 +39,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <thread>
 
 #include "bcmi.h"
 #include "bcmi_internal.h"
@@ -133,11 +134,11 @@ __device__ __forceinline__ void kt_exit(KT* kt) {
   if (!kt) return;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    atomicMax(&kt->end, (u64)wall_clock64());
-    unsigned prev = atomicAdd(&kt->count, 1u);
+    // Returning device-scope atomics complete at the coherence point before
+    // the next one issues, so the block that counts last sees every stamp.
+    u64 e_old = atomicMax(&kt->end, (u64)wall_clock64());
+    unsigned prev = atomicAdd(&kt->count, 1u + (unsigned)(e_old & 0));
     if (prev == gridDim.x - 1) {
-      __threadfence();
       u64 s0 = ld_coherent_u64(&kt->start), e0 = ld_coherent_u64(&kt->end);
       kt->total += e0 - s0;
       kt->launches += 1;
@@ -206,36 +207,101 @@ struct BA {
   int64_t n;
 };
 
-// Appends the vertices of the lanes with `want` set to the work list: each
-// gets ceil(deg/chunk) records.  Wave-wide call (every lane must call it).
-__device__ __forceinline__ int64_t wave_append(bool want, int32_t v, int64_t degv, int chunk, Ctl* c,
-                                               int32_t* rec_v, int32_t* rec_j, int64_t cap) {
-  int nch = want ? nchunks(degv, chunk) : 0;
-  int tot;
-  int pre = wave_excl_scan(nch, &tot);
-  if (tot == 0) return -1;
-  int lane = lane_id();
-  long long dsum = wave_sum_ll(want ? degv : 0);
-  int nv = __popcll(__ballot(want));
+// Work-list appends are staged per wave in LDS and flushed with one global
+// atomic per WBUF records, so the shared record counter is not hammered by
+// every wave iteration.  The frontier degree and vertex counts are kept in
+// registers and added once per wave at kernel end.
+#define WBUF 256
+#define MAX_WAVES_PER_BLOCK 4
+
+struct Appender {
+  int32_t* sv;  // this wave's LDS staging area
+  int32_t* sj;
+  int fill;     // wave-uniform
+  long long arcs, verts;  // per lane
+};
+
+__device__ __forceinline__ void app_init(Appender& a, int32_t (*lv)[WBUF], int32_t (*lj)[WBUF]) {
+  int w = threadIdx.x >> 6;
+  a.sv = lv[w];
+  a.sj = lj[w];
+  a.fill = 0;
+  a.arcs = 0;
+  a.verts = 0;
+}
+
+__device__ __forceinline__ void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void app_flush(Appender& a, Ctl* c, int32_t* rec_v, int32_t* rec_j, int64_t cap) {
+  if (a.fill == 0) return;
+  lds_wave_sync();
+  const int lane = lane_id();
   u64 base = 0;
-  if (lane == 0) {
-    base = atomicAdd((u64*)&c->rec_count, (u64)tot);
-    atomicAdd((u64*)&c->arcs_next, (u64)dsum);
-    atomicAdd((u64*)&c->verts_next, (u64)nv);
-  }
+  if (lane == 0) base = atomicAdd((u64*)&c->rec_count, (u64)a.fill);
   base = __shfl(base, 0, 64);
-  int64_t my = (int64_t)base + pre;
-  for (int k = 0; k < nch; k++) {
-    int64_t r = my + k;
+  for (int i = lane; i < a.fill; i += 64) {
+    int64_t r = (int64_t)base + i;
     if (r < cap) {
-      rec_v[r] = v;
-      rec_j[r] = k;
+      rec_v[r] = a.sv[i];
+      rec_j[r] = a.sj[i];
     } else {
       c->overflow = 1;
     }
   }
-  return want ? my : -1;
+  lds_wave_sync();
+  a.fill = 0;
 }
+
+// Appends the vertices of the lanes with `want` set: each gets
+// ceil(deg/chunk) records, contiguous.  Wave-wide call.
+__device__ __forceinline__ void app_push(Appender& a, bool want, int32_t v, int64_t degv, int chunk, Ctl* c,
+                                         int32_t* rec_v, int32_t* rec_j, int64_t cap) {
+  int nch = want ? nchunks(degv, chunk) : 0;
+  int tot;
+  int pre = wave_excl_scan(nch, &tot);
+  if (tot == 0) return;
+  if (want) { a.arcs += degv; a.verts += 1; }
+  if (a.fill + tot > WBUF) app_flush(a, c, rec_v, rec_j, cap);
+  if (tot > WBUF) {  // a hub: straight to global memory
+    const int lane = lane_id();
+    u64 base = 0;
+    if (lane == 0) base = atomicAdd((u64*)&c->rec_count, (u64)tot);
+    base = __shfl(base, 0, 64);
+    for (int k = 0; k < nch; k++) {
+      int64_t r = (int64_t)base + pre + k;
+      if (r < cap) {
+        rec_v[r] = v;
+        rec_j[r] = k;
+      } else {
+        c->overflow = 1;
+      }
+    }
+    return;
+  }
+  for (int k = 0; k < nch; k++) {
+    a.sv[a.fill + pre + k] = v;
+    a.sj[a.fill + pre + k] = k;
+  }
+  a.fill += tot;
+}
+
+__device__ __forceinline__ void app_finish(Appender& a, Ctl* c, int32_t* rec_v, int32_t* rec_j, int64_t cap) {
+  app_flush(a, c, rec_v, rec_j, cap);
+  long long ar = wave_sum_ll(a.arcs), vr = wave_sum_ll(a.verts);
+  if (lane_id() == 0) {
+    if (ar) atomicAdd((u64*)&c->arcs_next, (u64)ar);
+    if (vr) atomicAdd((u64*)&c->verts_next, (u64)vr);
+  }
+}
+
+#define APPENDER(a)                                        \
+  __shared__ int32_t app_lv_[MAX_WAVES_PER_BLOCK][WBUF];   \
+  __shared__ int32_t app_lj_[MAX_WAVES_PER_BLOCK][WBUF];   \
+  Appender a;                                              \
+  app_init(a, app_lv_, app_lj_)
 
 // ============================================================ plain sweep
 __global__ void k_plain_init(GA g, PA p) {
@@ -306,6 +372,7 @@ __device__ __forceinline__ void k_plain_td_body(GA g, PA p) {
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   long long acnt = 0;
   int ovf = 0;
+  APPENDER(ap);
   for (int64_t r = lo + gw; r < hi; r += tw) {
     int32_t u = p.rec_v[r];
     int j = p.rec_j[r];
@@ -335,9 +402,10 @@ __device__ __forceinline__ void k_plain_td_body(GA g, PA p) {
           dg = g.off[v + 1] - g.off[v];
         }
       }
-      wave_append(won, v, dg, g.chunk, c, p.rec_v, p.rec_j, p.cap);
+      app_push(ap, won, v, dg, g.chunk, c, p.rec_v, p.rec_j, p.cap);
     }
   }
+  app_finish(ap, c, p.rec_v, p.rec_j, p.cap);
   acnt = wave_sum_ll(acnt);
   if (lane == 0 && acnt) atomicAdd((u64*)&c->sum_a, (u64)acnt);
   if (ovf) c->sigma_ovf = 1;
@@ -361,6 +429,7 @@ __device__ __forceinline__ void k_plain_bu_body(GA g, PA p) {
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   long long acnt = 0;
   int ovf = 0;
+  APPENDER(ap);
   for (int64_t r = gw; r < g.nall; r += tw) {
     int32_t v = g.all_v[r];
     if (g.label[v] != lab) continue;
@@ -395,8 +464,9 @@ __device__ __forceinline__ void k_plain_bu_body(GA g, PA p) {
       if (won) p.delta[v] = g.w ? (double)g.w[v] : 0.0;
       acnt += cnt;
     }
-    wave_append(won, v, won ? g.off[v + 1] - g.off[v] : 0, g.chunk, c, p.rec_v, p.rec_j, p.cap);
+    app_push(ap, won, v, won ? g.off[v + 1] - g.off[v] : 0, g.chunk, c, p.rec_v, p.rec_j, p.cap);
   }
+  app_finish(ap, c, p.rec_v, p.rec_j, p.cap);
   acnt = wave_sum_ll(acnt);
   if (lane == 0 && acnt) atomicAdd((u64*)&c->sum_a, (u64)acnt);
   if (ovf) c->sigma_ovf = 1;
@@ -456,24 +526,13 @@ __device__ __forceinline__ void k_plain_back_body(GA g, PA p) {
     }
     acc = wave_sum_d(acc);
     if (lane == 0) {
-      double dl;
-      bool fin = true;
       if (nch == 1) {
-        dl = p.delta[v] + acc;
-      } else {
-        atomicAdd(&p.delta[v], acc);
-        __threadfence();
-        int t = atomicAdd(&p.done[v], 1);
-        fin = (t == nch - 1);
-        if (fin) {
-          p.done[v] = 0;
-          dl = ld_coherent(&p.delta[v]);
-        }
-      }
-      if (fin) {
+        double dl = p.delta[v] + acc;
         p.delta[v] = dl;
         p.coef[v] = (dl + 1.0) / sv;
         p.bc[v] += mult * dl;
+      } else {
+        atomicAdd(&p.delta[v], acc);  // finished by k_plain_bfin
       }
     }
   }
@@ -483,6 +542,25 @@ __global__ void __launch_bounds__(256) k_plain_back(GA g, PA p, KT* kt) {
   kt_enter(kt);
   k_plain_back_body(g, p);
   kt_exit(kt);
+}
+
+// Finishes the vertices of backward level d whose arcs were split over
+// several records (their partial sums were added atomically).
+__global__ void k_plain_bfin(GA g, PA p) {
+  Ctl* c = p.ctl;
+  if (!c->fdone || c->bdone) return;
+  const int d = c->blevel;
+  const int64_t lo = p.rec_lo[d], hi = p.rec_lo[d + 1];
+  const double mult = p.multbuf[0];
+  for (int64_t r = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < hi; r += (int64_t)gridDim.x * blockDim.x) {
+    if (p.rec_j[r] != 0) continue;
+    int32_t v = p.rec_v[r];
+    int64_t deg = g.off[v + 1] - g.off[v];
+    if (deg <= g.chunk) continue;
+    double dl = p.delta[v];
+    p.coef[v] = (dl + 1.0) / (double)p.sigma[v];
+    p.bc[v] += mult * dl;
+  }
 }
 
 __global__ void k_plain_cleanup(PA p) {
@@ -610,6 +688,7 @@ __device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
   const int lane = lane_id();
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  APPENDER(ap);
   for (int64_t r = lo + gw; r < hi; r += tw) {
     int32_t u = b.rec_v[r];
     int j = b.rec_j[r];
@@ -630,10 +709,11 @@ __device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
           if (won) dg = g.off[v + 1] - g.off[v];
         }
       }
-      wave_append(won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
+      app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
       zero_multi_sigma(won && dg > g.chunk, v, b);
     }
   }
+  app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
 }
 
 __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
@@ -653,6 +733,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   const int lane = lane_id();
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  APPENDER(ap);
   for (int64_t r = gw; r < g.nall; r += tw) {
     int32_t v = g.all_v[r];
     u64 sv = b.seen[v];
@@ -676,9 +757,10 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         if (won) dg = g.off[v + 1] - g.off[v];
       }
     }
-    wave_append(won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
+    app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
     zero_multi_sigma(won && dg > g.chunk, v, b);
   }
+  app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
 }
 
 __global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
@@ -731,17 +813,31 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
         F = cur[u] & N;
       }
       u64 P = __ballot(F != 0);
+      // Parents four at a time: the four row loads are issued before any is
+      // consumed, and the sums still follow adjacency order.
       while (P) {
-        int q = __ffsll((long long)P) - 1;
-        P &= P - 1;
-        int32_t uq = rl32(u, q);
-        u64 fq = rl64(F, q);
-        if ((fq >> lane) & 1) {
-          u64 su = b.sigma[(int64_t)uq * 64 + lane];
-          u64 s = acc + su;
+        int q[4];
+        bool ok[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          ok[t] = P != 0;
+          q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
+          if (ok[t]) P &= P - 1;
+        }
+        u64 x[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          int32_t uq = rl32(u, q[t]);
+          u64 fq = rl64(F, q[t]);
+          bool use = ok[t] && ((fq >> lane) & 1);
+          x[t] = use ? b.sigma[(int64_t)uq * 64 + lane] : 0;
+          ca += use;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          u64 s = acc + x[t];
           if (s < acc) ovf = 1;
           acc = s;
-          ca++;
         }
       }
     }
@@ -827,35 +923,40 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
         M = lmn[z] & L;
       }
       u64 P = __ballot(M != 0);
+      // Successors four at a time (loads first, then the sums in adjacency
+      // order; FP contraction is off, so each step rounds like the oracle).
       while (P) {
-        int q = __ffsll((long long)P) - 1;
-        P &= P - 1;
-        int32_t zq = rl32(z, q);
-        u64 mq = rl64(M, q);
-        if ((mq >> lane) & 1) acc = acc + b.coef[(int64_t)zq * 64 + lane] * sv;
+        int q[4];
+        bool ok[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          ok[t] = P != 0;
+          q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
+          if (ok[t]) P &= P - 1;
+        }
+        double x[4];
+        bool use[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          int32_t zq = rl32(z, q[t]);
+          u64 mq = rl64(M, q[t]);
+          use[t] = ok[t] && ((mq >> lane) & 1);
+          x[t] = use[t] ? b.coef[(int64_t)zq * 64 + lane] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          if (use[t]) acc = acc + x[t] * sv;
       }
     }
     double* crow = b.coef + (int64_t)v * 64 + lane;
-    bool fin = true;
-    double dl = acc;
     if (nch > 1) {
-      if (mine) atomicAdd(crow, acc);
-      __threadfence();
-      int t = 0;
-      if (lane == 0) t = atomicAdd(&b.done[v], 1);
-      t = __shfl(t, 0, 64);
-      fin = (t == nch - 1);
-      if (fin) {
-        if (lane == 0) b.done[v] = 0;
-        dl = mine ? wv + ld_coherent(crow) : 0.0;
-      }
-    }
-    if (fin) {
+      if (mine) atomicAdd(crow, acc);  // finished by k_ms_bfin
+    } else {
       double contrib = 0.0;
       if (mine) {
-        *crow = (dl + 1.0) / sv;
-        contrib = mult * dl;
-        if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = dl;
+        *crow = (acc + 1.0) / sv;
+        contrib = mult * acc;
+        if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = acc;
       }
       contrib = wave_sum_d(contrib);
       if (lane == 0) b.bc[v] += contrib;
@@ -876,6 +977,37 @@ __global__ void __launch_bounds__(256) k_ms_back(GA g, BA b, KT* kt) {
   kt_enter(kt);
   k_ms_back_body(g, b);
   kt_exit(kt);
+}
+
+// Finishes the level-d vertices whose arcs were split over several records:
+// delta = w(v) + the atomically added partial sums held in coef.
+__global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (!c->fdone || c->bdone) return;
+  const int d = c->blevel;
+  const int64_t lo = b.rec_lo[d], hi = b.rec_lo[d + 1];
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  const double mult = b.multbuf[lane];
+  for (int64_t r = lo + gw; r < hi; r += tw) {
+    if (b.rec_j[r] != 0) continue;
+    int32_t v = b.rec_v[r];
+    int64_t deg = g.off[v + 1] - g.off[v];
+    if (deg <= g.chunk) continue;
+    u64 L = b.rec_mask[r];
+    const bool mine = (L >> lane) & 1;
+    double contrib = 0.0;
+    if (mine) {
+      double* crow = b.coef + (int64_t)v * 64 + lane;
+      double dl = (g.w ? (double)g.w[v] : 0.0) + *crow;
+      *crow = (dl + 1.0) / (double)b.sigma[(int64_t)v * 64 + lane];
+      contrib = mult * dl;
+      if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = dl;
+    }
+    contrib = wave_sum_d(contrib);
+    if (lane == 0) b.bc[v] += contrib;
+  }
 }
 
 __global__ void k_ms_cleanup(GA g, BA b) {
@@ -1107,7 +1239,52 @@ void DevGraph::release() {
   fr(round); fr(parent); fr(w); fr(root); fr(order);
   off = foff = toff = nullptr; adj = fadj = tadj = nullptr; label = nullptr; comp_size = comp_arcs = nullptr;
   all_v = all_j = nullptr; round = parent = root = order = nullptr; w = nullptr;
-  n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); bytes = 0;
+  n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); h_sig.clear(); bytes = 0;
+}
+
+// Batching key per vertex: BFS depths (8 bits each, 255 = unreached) from the
+// four highest-degree vertices.  Sources with equal keys have similar depth
+// profiles, so a batch of them shares levels: its per-vertex masks are dense
+// and each vertex appears on few levels.
+static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std::vector<uint32_t>& sig) {
+  const int H = 4;
+  std::vector<int32_t> hubs;
+  {
+    std::vector<std::pair<int64_t, int32_t>> best;
+    for (int64_t v = 0; v < n; v++) {
+      int64_t d = off[v + 1] - off[v];
+      if ((int)best.size() < H || d > best.back().first) {
+        best.push_back({d, (int32_t)v});
+        std::sort(best.begin(), best.end(), [](auto& a, auto& b) { return a.first > b.first; });
+        if ((int)best.size() > H) best.pop_back();
+      }
+    }
+    for (auto& x : best) hubs.push_back(x.second);
+  }
+  std::vector<std::vector<uint8_t>> dep(H, std::vector<uint8_t>(n, 255));
+  std::vector<std::thread> th;
+  for (int h = 0; h < (int)hubs.size(); h++)
+    th.emplace_back([&, h] {
+      std::vector<uint8_t>& D = dep[h];
+      std::vector<int32_t> q;
+      q.reserve(1024);
+      q.push_back(hubs[h]);
+      D[hubs[h]] = 0;
+      for (size_t i = 0; i < q.size(); i++) {
+        int32_t u = q[i];
+        uint8_t du = D[u];
+        if (du >= 254) continue;
+        for (int64_t e = off[u]; e < off[u + 1]; e++)
+          if (D[adj[e]] == 255) { D[adj[e]] = du + 1; q.push_back(adj[e]); }
+      }
+    });
+  for (auto& t : th) t.join();
+  sig.assign(n, 0);
+  for (int64_t v = 0; v < n; v++) {
+    uint32_t k = 0;
+    for (int h = 0; h < H; h++) k = (k << 8) | (h < (int)hubs.size() ? dep[h][v] : 0);
+    sig[v] = k;
+  }
 }
 
 static void chunk_records(const std::vector<int64_t>& off, int chunk, std::vector<int32_t>& rv,
@@ -1238,6 +1415,7 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
     HIP_TRY(hipStreamSynchronize(st));
   }
   g.h_off = hoff;
+  hub_signature(n, off, adj, g.h_sig);
   g.h_round.assign(n, 0);
   if (heuristic) {
     HIP_TRY(hipMemcpyAsync(g.h_round.data(), g.round, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
@@ -1502,6 +1680,10 @@ int workspace_bc_reset(Workspace* ws) {
   return BCMI_OK;
 }
 
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 // Captures `steps` forward levels and `steps` backward levels into two
 // graphs (kernel nodes only).
 static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
@@ -1527,6 +1709,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
         } else {
           hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, pa.ctl);
           hipLaunchKernelGGL(k_plain_back, G, B, 0, st, ga, pa, tk(7));
+          hipLaunchKernelGGL(k_plain_bfin, G, B, 0, st, ga, pa);
         }
       } else {
         BA ba = make_ba(ws, prm);
@@ -1538,6 +1721,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
         } else {
           hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, ba.ctl);
           hipLaunchKernelGGL(k_ms_back, G, B, 0, st, ga, ba, tk(2));
+          hipLaunchKernelGGL(k_ms_bfin, G, B, 0, st, ga, ba);
         }
       }
     }
@@ -1574,6 +1758,29 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
     } else {
       BA ba = make_ba(ws, prm);
       hipLaunchKernelGGL(k_ms_init, 1, 64, 0, st, ga, ba, k);
+    }
+    if (prm.trace) {
+      // Diagnostic: one graph launch at a time, level statistics on stderr.
+      HIP_TRY(hipStreamSynchronize(st));
+      for (int it = 0; it < 100000; it++) {
+        double t0 = now_ms();
+        HIP_TRY(hipGraphLaunch(ws->gfwd, st));
+        HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const Ctl& c = *ws->hctl;
+        fprintf(stderr, "[trace] fwd level %d mode %s verts %lld arcs %lld pending-arcs %lld %.3f ms%s\n", c.level,
+                c.mode ? "BU" : "TD", (long long)c.verts_cur, (long long)c.arcs_cur,
+                (long long)(c.arcs_reach - c.arcs_seen), now_ms() - t0, c.fdone ? " (done)" : "");
+        if (c.fdone) break;
+      }
+      for (int it = 0; it < 100000; it++) {
+        double t0 = now_ms();
+        HIP_TRY(hipGraphLaunch(ws->gbwd, st));
+        HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        fprintf(stderr, "[trace] bwd level %d %.3f ms\n", ws->hctl->blevel, now_ms() - t0);
+        if (ws->hctl->bdone) break;
+      }
     }
     HIP_TRY(hipGraphLaunch(ws->gfwd, st));
     HIP_TRY(hipGraphLaunch(ws->gbwd, st));
@@ -1684,10 +1891,6 @@ static int prepare_sources(const DevGraph& g, Workspace* ws, const int32_t* src,
   return BCMI_OK;
 }
 
-static double now_ms() {
-  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
 int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* sources, int64_t nsrc, int mode,
            int src_flags, RunResult* out) {
   if (mode != ws->mode) return set_error(BCMI_ERR_STATE, "workspace was created for another mode");
@@ -1723,6 +1926,19 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
     }
   } else {
     int W = std::max(1, std::min(64, prm.width));
+    // Batch sources with similar depth profiles together (BC is a sum over
+    // sources, so the order only changes the rounding of the final sums).
+    {
+      std::vector<size_t> idx(cs.size());
+      for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+      const std::vector<uint32_t>& sg = g.h_sig;
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return sg[cs[a]] < sg[cs[b]]; });
+      std::vector<int32_t> cs2(cs.size());
+      std::vector<double> cm2(cm.size());
+      for (size_t i = 0; i < idx.size(); i++) { cs2[i] = cs[idx[i]]; cm2[i] = cm[idx[i]]; }
+      cs.swap(cs2);
+      cm.swap(cm2);
+    }
     for (size_t i = 0; i < cs.size(); i += W) {
       int k = (int)std::min<size_t>(W, cs.size() - i);
       for (int q = 0; q < 64; q++) {

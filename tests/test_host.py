@@ -1,0 +1,139 @@
+"""CPU tests of the engine's host side: graph ingest and CSR build against the
+oracle's restatement of the reference pipeline, the synthetic generators, and
+the C ABI (library loads, exports every declared entry point, fails cleanly
+without a GPU).  No device calls."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import bcmi
+from oracle import oracle as O
+import graphs as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "bcmi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bcmi_\w+)\s*\(", src)))
+
+
+def test_abi_exports_every_declared_symbol():
+    lib = bcmi.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(bcmi.EXPORTS) == set(names)
+
+
+def test_engine_without_gpu_fails_cleanly():
+    import subprocess, sys
+    code = ("import sys; sys.path.insert(0, %r); import bcmi\n"
+            "try:\n  bcmi.Engine()\nexcept bcmi.BcmiError as e:\n  print('ERR', e.code)\n"
+            "else:\n  print('GPU')\n") % os.path.join(ROOT, "ck-bc_amd")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120).stdout
+    assert out.strip() in ("GPU", "ERR %d" % bcmi.ERR_HIP)
+
+
+def test_last_error_and_bad_args():
+    with pytest.raises(bcmi.BcmiError) as ex:
+        bcmi.Graph.rmat(0, 16)
+    assert ex.value.code == bcmi.ERR_ARG
+    with pytest.raises(bcmi.BcmiError) as ex:
+        bcmi.Graph.from_edges([[0, 5]], 3)
+    assert ex.value.code == bcmi.ERR_RANGE
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_csr_matches_reference_pipeline(directed):
+    for seed in range(5):
+        e, n = G.random_graph(200, 700, seed)
+        g = bcmi.Graph.from_edges(e, n, directed=directed)
+        off, adj = O.csr_from_edges(e, n, directed=directed)
+        assert np.array_equal(g.off, off) and np.array_equal(g.adj, adj)
+
+
+def test_read_edgelist_facebook(tmp_path):
+    f = np.load(os.path.join(GOLD, "facebook.npz"))
+    n = int(f["n"])
+    e = f["edges"].astype(np.int64)
+    fn = tmp_path / "fb.txt"
+    with open(fn, "w") as fp:
+        fp.write("# Directed graph\n# Nodes: 4039 Edges: 88234\n")
+        for a, b in e:
+            fp.write(f"{a} {b}\n")
+    g = bcmi.Graph.read_edgelist(str(fn), n)
+    assert g.header == (4039, 88234)
+    arcs = O.normalize(O.read_edges(str(fn), n))
+    off, adj = O.build_csc(arcs, n)
+    assert np.array_equal(g.off, off) and np.array_equal(g.adj, adj)
+    assert g.m == 2 * 88234
+
+
+def test_read_edgelist_quirks(tmp_path):
+    fn = tmp_path / "g.txt"
+    fn.write_text("# Nodes: 5 Edges: 6\n0 1\n1 2\n\n2 0\n# comment 3 4\n3 3\n1 0\n4 2\n")
+    g = bcmi.Graph.read_edgelist(str(fn), 5)
+    off, adj = O.build_csc(O.normalize(O.read_edges(str(fn), 5)), 5)
+    assert np.array_equal(g.off, off) and np.array_equal(g.adj, adj)
+    gd = bcmi.Graph.read_edgelist(str(fn), 5, directed=True)
+    offd, adjd = O.build_csc(O.normalize(O.read_edges(str(fn), 5, directed=True)), 5)
+    assert np.array_equal(gd.off, offd) and np.array_equal(gd.adj, adjd)
+    with pytest.raises(bcmi.BcmiError) as ex:
+        bcmi.Graph.read_edgelist(str(fn), 4)
+    assert ex.value.code == bcmi.ERR_RANGE and "line 7" in str(ex.value)
+    with pytest.raises(bcmi.BcmiError) as ex:
+        bcmi.Graph.read_edgelist(str(tmp_path / "missing.txt"), 4)
+    assert ex.value.code == bcmi.ERR_IO
+
+
+def test_rmat_generator_is_stable_and_well_formed():
+    f = np.load(os.path.join(GOLD, "rmat10.npz"))
+    g = bcmi.Graph.rmat(10, 16, seed=20240101)
+    src = np.repeat(np.arange(g.n), np.diff(g.off))
+    keep = src < g.adj
+    assert np.array_equal(np.stack([src[keep], g.adj[keep]], 1), f["edges"].astype(np.int64))
+    g2 = bcmi.Graph.rmat(12, 8, seed=3)
+    s2 = np.repeat(np.arange(g2.n), np.diff(g2.off))
+    assert np.all(s2 != g2.adj)                         # no self loops
+    pairs = set(zip(s2.tolist(), g2.adj.tolist()))
+    assert all((b, a) in pairs for a, b in pairs)       # symmetric
+    assert all(np.all(np.diff(g2.adj[g2.off[v]:g2.off[v + 1]]) > 0) for v in range(0, g2.n, 97))
+    deg = g2.degrees()
+    assert deg.max() > 20 * deg[deg > 0].mean()         # skewed degrees (a = .57)
+    assert g2.m <= 2 * 8 * g2.n
+
+
+def test_pendant_trees_generator():
+    core = bcmi.Graph.rmat(10, 16, seed=5)
+    g = core.with_pendant_trees(0.3, seed=9)
+    extra = g.n - core.n
+    assert extra == round(0.3 * core.n)
+    assert g.m == core.m + 2 * extra                    # each new vertex adds one edge
+    p = O.peel(g.off, g.adj)
+    removed = int((p["round"] > 0).sum())
+    assert removed >= extra
+    assert 1 <= p["rounds"] <= 8
+
+
+def test_block_chain_generator():
+    g = bcmi.Graph.block_chain(10, 64, 8.0, seed=2)
+    assert g.n == 640
+    src = np.repeat(np.arange(g.n), np.diff(g.off))
+    cross = (src // 64) != (g.adj // 64)
+    assert int(cross.sum()) == 2 * 9                    # one bridge between consecutive blocks
+    assert np.all(np.abs(src[cross] // 64 - g.adj[cross] // 64) == 1)
+
+
+def test_sample_sources():
+    g = bcmi.Graph.rmat(11, 4, seed=1)
+    s = g.sample_sources(500, seed=7)
+    assert len(set(s.tolist())) == 500
+    assert np.all(g.degrees()[s] > 0)
+    assert np.array_equal(s, g.sample_sources(500, seed=7))
+    assert not np.array_equal(s, g.sample_sources(500, seed=8))
