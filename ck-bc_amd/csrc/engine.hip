@@ -48,6 +48,10 @@
 namespace bcmi {
 
 static thread_local std::string g_err;
+// Graph capture must not overlap device allocation or freeing in another
+// thread (HIP invalidates the capture), so both take this lock.
+static std::recursive_mutex g_alloc_mu;
+#define ALLOC_LOCK std::lock_guard<std::recursive_mutex> alloc_lock_(g_alloc_mu)
 int set_error(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -1232,6 +1236,7 @@ static int dalloc(T** p, size_t count, size_t* acc = nullptr) {
   } while (0)
 
 void DevGraph::release() {
+  ALLOC_LOCK;
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   if (toff != off) { fr(toff); fr(tadj); }
   if (off != foff) { fr(off); fr(adj); }
@@ -1301,6 +1306,7 @@ static void chunk_records(const std::vector<int64_t>& off, int chunk, std::vecto
 
 int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* adj, int directed,
                    int heuristic, const Params& prm, hipStream_t st) {
+  ALLOC_LOCK;
   g.release();
   if (n < 1 || n > INT32_MAX) return set_error(BCMI_ERR_ARG, "vertex count out of range");
   if (off[0] != 0) return set_error(BCMI_ERR_ARG, "offsets must start at 0");
@@ -1569,6 +1575,7 @@ __global__ void k_kt_reset(KT* kt) {
 }
 
 void workspace_destroy(Workspace* ws) {
+  ALLOC_LOCK;
   if (!ws) return;
   free_graphs(ws);
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
@@ -1590,6 +1597,7 @@ void workspace_destroy(Workspace* ws) {
 }
 
 static int alloc_records(Workspace* ws, int64_t cap) {
+  ALLOC_LOCK;
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   fr(ws->rec_v); fr(ws->rec_j); fr(ws->rec_mask);
   ws->rec_v = nullptr; ws->rec_j = nullptr; ws->rec_mask = nullptr;
@@ -1657,6 +1665,7 @@ static int ws_init(Workspace* ws, const DevGraph& g, int mode, int width, const 
 }
 
 Workspace* workspace_create(const DevGraph& g, int mode, int width, const Params& prm, int* rc) {
+  ALLOC_LOCK;
   Workspace* ws = new Workspace();
   *rc = ws_init(ws, g, mode, width, prm);
   if (*rc != BCMI_OK) {
@@ -1668,6 +1677,7 @@ Workspace* workspace_create(const DevGraph& g, int mode, int width, const Params
 hipStream_t workspace_stream(Workspace* ws) { return ws->st; }
 size_t workspace_size(Workspace* ws) { return ws->bytes; }
 void workspace_share_bc(Workspace* ws, double* bc) {
+  ALLOC_LOCK;
   if (ws->own_bc && ws->bc) (void)hipFree(ws->bc);
   ws->bc = bc;
   ws->own_bc = false;
@@ -1687,6 +1697,7 @@ static double now_ms() {
 // Captures `steps` forward levels and `steps` backward levels into two
 // graphs (kernel nodes only).
 static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
+  ALLOC_LOCK;
   free_graphs(ws);
   GA ga = make_ga(g);
   hipStream_t st = ws->st;
@@ -1862,6 +1873,7 @@ static int prepare_sources(const DevGraph& g, Workspace* ws, const int32_t* src,
   hipStream_t st = ws->st;
   int64_t n = g.n;
   if (nsrc > ws->srcdev_cap) {
+    ALLOC_LOCK;
     if (ws->srcdev) (void)hipFree(ws->srcdev);
     ws->srcdev = nullptr;
     TRY(dalloc(&ws->srcdev, nsrc));
@@ -1977,7 +1989,8 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   HIP_TRY(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
   R.gpu_ms = ms;
   R.wall_ms = now_ms() - t0;
-  HIP_TRY(hipMemcpy(ws->hkt, ws->kt, sizeof(KT), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(ws->hkt, ws->kt, sizeof(KT), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
   {
     int dev = 0, khz = 0;
     HIP_TRY(hipGetDevice(&dev));
@@ -2040,6 +2053,7 @@ int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32
   hipStream_t st = ws->st;
   int64_t n = g.n;
   if (!ws->dbg_delta) {
+    ALLOC_LOCK;
     TRY(dalloc(&ws->dbg_delta, 64 * n));
     free_graphs(ws);
   }
