@@ -59,7 +59,19 @@ def main():
     ap.add_argument("--seed", type=int, default=20250101)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="back", choices=["back", "sigma", "td", "bu"])
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (PMC byte calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy} MiB five times")
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -100,17 +112,15 @@ def main():
     log(f"batched: {len(timed)} sources in {t_batched:.3f}s -> {gteps:.2f} GTEPS, "
         f"model bytes {st['bytes'] / 1e9:.1f} GB ({st['bytes'] / t_batched / 1e9:.0f} GB/s whole-run)")
 
-    # dominant kernel roofline: per-kernel share of the byte model
-    n_s, m_s, a_s = ss["n_s"].astype(np.float64), ss["m_s"].astype(np.float64), ss["a_s"].astype(np.float64)
-    uni_m = st["sum_m"]  # placeholder, replaced below by per-batch union counts if available
-    # Byte model split (batched, per batch S): forward pass = CSR once (8 n_S + 4 m_S) +
-    # 8 m_S mask probes + sum_s (8 a_s + 16 n_s); backward = the same CSR and probes +
-    # sum_s (16 a_s + 16 n_s) + 16 n_S (BC RMW).
-    bwd_share = (12.0 * 0 + 0.0)
-    batches = max(1, int(st["batches"]))
-    # union reach per batch is reported through total bytes: B = 24 m_S + 32 n_S + sum(24 a + 32 n)
+    # Dominant-kernel roofline.  The batched byte model per batch is
+    # B(S) = 24 m_S + 32 n_S + sum_s (24 a_s + 32 n_s); each of the two passes
+    # (forward, backward) reads the CSR and probes masks once, so each gets half
+    # of the union part; the backward pass gets sum_s (16 a_s + 16 n_s)
+    # (coef of successors + sigma/coef of the vertex itself), the sigma pass
+    # sum_s (8 a_s + 16 n_s).
+    n_s, a_s = ss["n_s"].astype(np.float64), ss["a_s"].astype(np.float64)
     per_src = float(np.sum(24.0 * a_s + 32.0 * n_s))
-    union_part = st["bytes"] - per_src  # = sum over batches of (24 m_S + 32 n_S)
+    union_part = st["bytes"] - per_src
     if args.kernel == "back":
         kbytes = union_part / 2.0 + float(np.sum(16.0 * a_s + 16.0 * n_s))
     elif args.kernel == "sigma":
@@ -132,13 +142,8 @@ def main():
     sync()
     t_plain = time.perf_counter() - t0
     gteps_plain = stp["te"] / t_plain / 1e9
-    # batched GTEPS on exactly these sources, for the like-for-like ratio
-    e.reset()
-    t0 = time.perf_counter()
-    stb = e.run(plain_src, mode=bcmi.MODE_BATCHED)
-    t_b2 = time.perf_counter() - t0
-    log(f"plain: {len(plain_src)} sources in {t_plain:.3f}s -> {gteps_plain:.2f} GTEPS "
-        f"(batched on the same sources {stb['te'] / t_b2 / 1e9:.2f} GTEPS)")
+    log(f"plain: {len(plain_src)} sources (the first of the timed list) in {t_plain:.3f}s "
+        f"-> {gteps_plain:.2f} GTEPS")
 
     cpu = None
     if not args.no_cpu:

@@ -2,7 +2,10 @@
 // workspaces sharing one BC accumulator).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <climits>
 #include <cstring>
+#include <vector>
 #include <mutex>
 #include <string>
 
@@ -115,6 +118,9 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "width") {
     if (v < 1 || v > 64) return set_error(BCMI_ERR_ARG, "width must be in [1,64]");
     p.width = (int)v;
+  } else if (k == "reorder") {
+    if (c->have_graph) return set_error(BCMI_ERR_STATE, "reorder must be set before uploading a graph");
+    p.reorder = v ? 1 : 0;
   } else if (k == "trace") {
     p.trace = v ? 1 : 0;
   } else if (k == "ktime") {
@@ -140,6 +146,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "width") *v = p.width;
   else if (k == "ktime") *v = p.ktime;
   else if (k == "trace") *v = p.trace;
+  else if (k == "reorder") *v = p.reorder;
   else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
   return BCMI_OK;
 }
@@ -175,6 +182,14 @@ int bcmi_get_peel(bcmi_ctx* c, int32_t* round, int32_t* parent, uint64_t* w, int
   if (parent && e == hipSuccess) e = hipMemcpy(parent, c->g.parent, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
   if (w && e == hipSuccess) e = hipMemcpy(w, c->g.w, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  if (round) c->g.to_caller(round);
+  if (w) c->g.to_caller(w);
+  if (parent) {
+    c->g.to_caller(parent);
+    if (c->g.relabeled)
+      for (int64_t v = 0; v < n; v++)
+        if (parent[v] >= 0) parent[v] = c->g.h_n2o[parent[v]];
+  }
   if (nrounds) *nrounds = c->g.nrounds;
   return BCMI_OK;
 }
@@ -186,6 +201,15 @@ int bcmi_get_labels(bcmi_ctx* c, int32_t* label) {
   use_device(c);
   hipError_t e = hipMemcpy(label, c->g.label, sizeof(int32_t) * c->g.n, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  if (c->g.relabeled) {
+    // internal labels are the smallest internal id; report the smallest caller id
+    int64_t n = c->g.n;
+    std::vector<int32_t> mn(n, INT32_MAX);
+    for (int64_t i = 0; i < n; i++) mn[label[i]] = std::min(mn[label[i]], c->g.h_n2o[i]);
+    std::vector<int32_t> out(n);
+    for (int64_t v = 0; v < n; v++) out[v] = mn[label[c->g.h_o2n[v]]];
+    memcpy(label, out.data(), sizeof(int32_t) * n);
+  }
   return BCMI_OK;
 }
 
@@ -236,6 +260,7 @@ int bcmi_get_bc(bcmi_ctx* c, double* bc) {
   use_device(c);
   hipError_t e = hipMemcpy(bc, c->bc, sizeof(double) * c->g.n, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  c->g.to_caller(bc);
   return BCMI_OK;
 }
 

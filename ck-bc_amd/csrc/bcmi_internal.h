@@ -3,6 +3,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "bcmi.h"
 
@@ -14,6 +15,8 @@ int set_error(int code, const std::string& msg);
 bcmi_graph* graph_alloc(int64_t n, int64_t m, int directed);
 bcmi_graph* csr_from_arcs(int64_t n, const int32_t* tail, const int32_t* head, int64_t k, int mirror,
                           int directed);
+void relabel_by_degree(int64_t n, const int64_t* off, const int32_t* adj, std::vector<int32_t>& n2o,
+                       std::vector<int32_t>& o2n, std::vector<int64_t>& noff, std::vector<int32_t>& nadj);
 
 // Tunable launch and algorithm parameters.
 struct Params {
@@ -24,6 +27,7 @@ struct Params {
   int64_t alpha_batched = 1000; // x1000
   int steps = 16;           // levels per captured graph
   int width = 64;           // sources per batch
+  int reorder = 1;          // relabel vertices by descending degree at upload (locality)
   int trace = 0;            // diagnostic: per-level statistics on stderr (use with steps = 1)
   int ktime = 0;            // kernel timed with HIP events inside the level loop:
                             // 0 none, 1 batched sigma, 2 batched backward, 3 batched top-down,

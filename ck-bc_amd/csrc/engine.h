@@ -70,6 +70,19 @@ struct DevGraph {
   std::vector<int64_t> h_off;    // traversed offsets on the host (degrees)
   std::vector<int32_t> h_round;  // removal round per vertex on the host
   std::vector<uint32_t> h_sig;   // batching key: BFS depths from the four largest hubs
+  // Vertices are relabelled by descending degree on upload (hubs' rows and
+  // masks then share cache lines); the caller's ids are mapped at the ABI.
+  bool relabeled = false;
+  std::vector<int32_t> h_n2o, h_o2n;  // internal id -> caller id, and back
+  int32_t* d_n2o = nullptr;
+  int32_t to_int(int32_t v) const { return relabeled ? h_o2n[v] : v; }
+  // out[v] = in[internal(v)] for every caller id v (in-place).
+  template <class T>
+  void to_caller(T* a) const {
+    if (!relabeled) return;
+    std::vector<T> t(a, a + n);
+    for (int64_t v = 0; v < n; v++) a[v] = t[h_o2n[v]];
+  }
   size_t bytes = 0;
   void release();
 };

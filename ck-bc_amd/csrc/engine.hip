@@ -1109,7 +1109,7 @@ __global__ void k_peel_mark(const int32_t* list, int64_t nl, const int32_t* deg,
 // smaller end).  Removed vertices go to order[base..].
 __global__ void k_peel_decide(const int64_t* off, const int32_t* adj, const int32_t* list, int64_t nl,
                               const int32_t* round, const int32_t* cand, int r, int32_t* parent,
-                              int32_t* order, u64* ocnt) {
+                              int32_t* order, u64* ocnt, const int32_t* n2o) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
     int32_t u = list[i];
     if (cand[u] != r) continue;
@@ -1117,7 +1117,7 @@ __global__ void k_peel_decide(const int64_t* off, const int32_t* adj, const int3
     for (int64_t e = off[u]; e < off[u + 1]; e++)
       if (round[adj[e]] == 0) { p = adj[e]; break; }
     if (p < 0) continue;
-    if (cand[p] == r && p > u) continue;
+    if (cand[p] == r && (n2o ? n2o[p] > n2o[u] : p > u)) continue;  // ties in the caller's ids
     parent[u] = p;
     order[atomicAdd(ocnt, 1ull)] = u;
   }
@@ -1241,7 +1241,8 @@ void DevGraph::release() {
   if (toff != off) { fr(toff); fr(tadj); }
   if (off != foff) { fr(off); fr(adj); }
   fr(foff); fr(fadj); fr(label); fr(comp_size); fr(comp_arcs); fr(all_v); fr(all_j);
-  fr(round); fr(parent); fr(w); fr(root); fr(order);
+  fr(round); fr(parent); fr(w); fr(root); fr(order); fr(d_n2o);
+  d_n2o = nullptr; relabeled = false; h_n2o.clear(); h_o2n.clear();
   off = foff = toff = nullptr; adj = fadj = tadj = nullptr; label = nullptr; comp_size = comp_arcs = nullptr;
   all_v = all_j = nullptr; round = parent = root = order = nullptr; w = nullptr;
   n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); h_sig.clear(); bytes = 0;
@@ -1316,6 +1317,14 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
   for (int64_t e = 0; e < m; e++)
     if (adj[e] < 0 || adj[e] >= n) return set_error(BCMI_ERR_RANGE, "adjacency entry out of range");
   if (heuristic && directed) return set_error(BCMI_ERR_ARG, "degree-1 peeling needs an undirected graph");
+  std::vector<int64_t> roff;
+  std::vector<int32_t> radj;
+  if (prm.reorder) {
+    relabel_by_degree(n, off, adj, g.h_n2o, g.h_o2n, roff, radj);
+    off = roff.data();
+    adj = radj.data();
+    g.relabeled = true;
+  }
   g.n = n; g.fm = m; g.directed = directed; g.heuristic = heuristic; g.chunk = prm.chunk;
   size_t acc = 0;
   TRY(dalloc(&g.foff, n + 1, &acc));
@@ -1330,6 +1339,10 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
   TRY(dalloc(&g.w, n, &acc));
   TRY(dalloc(&g.root, n, &acc));
   TRY(dalloc(&g.order, n, &acc));
+  if (g.relabeled) {
+    TRY(dalloc(&g.d_n2o, n, &acc));
+    HIP_TRY(hipMemcpyAsync(g.d_n2o, g.h_n2o.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  }
   const int B = 256;
   int G = grid_for(n);
   hipLaunchKernelGGL(k_fill_i32, G, B, 0, st, g.round, 0, n);
@@ -1359,7 +1372,7 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
       hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, st, cnts + 1, 0ull, (int64_t)1);
       hipLaunchKernelGGL(k_peel_mark, Gl, B, 0, st, la, nl, deg, g.round, cand, r);
       hipLaunchKernelGGL(k_peel_decide, Gl, B, 0, st, g.foff, g.fadj, la, nl, g.round, cand, r, g.parent,
-                         g.order, cnts + 2);
+                         g.order, cnts + 2, g.d_n2o);
       HIP_TRY(hipMemcpyAsync(hc + 2, cnts + 2, sizeof(u64), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
       int64_t nrem = (int64_t)hc[2] - ocount;
@@ -1864,6 +1877,12 @@ static int prepare_sources(const DevGraph& g, Workspace* ws, const int32_t* src,
   const std::vector<int64_t>& ho = g.h_off;
   for (int64_t i = 0; i < nsrc; i++)
     if (src[i] < 0 || src[i] >= g.n) return set_error(BCMI_ERR_RANGE, "source id out of range");
+  std::vector<int32_t> mapped;
+  if (g.relabeled) {
+    mapped.resize(nsrc);
+    for (int64_t i = 0; i < nsrc; i++) mapped[i] = g.h_o2n[src[i]];
+    src = mapped.data();
+  }
   if (!g.heuristic) {
     for (int64_t i = 0; i < nsrc; i++)
       if (ho[src[i] + 1] > ho[src[i]]) { cs.push_back(src[i]); cm.push_back(1.0); }
@@ -2023,6 +2042,8 @@ int sssp_debug(const DevGraph& g, Workspace* ws, const Params& prm, int32_t s, i
   if (ws->mode != BCMI_MODE_PLAIN) return set_error(BCMI_ERR_STATE, "plain workspace required");
   if (s < 0 || s >= g.n) return set_error(BCMI_ERR_RANGE, "source out of range");
   hipStream_t st = ws->st;
+  const int32_t s_caller = s;
+  s = g.to_int(s);
   // Sweep into a scratch BC (bc unchanged): the multiplier is 0.
   ws->hsrc[0] = s;
   ws->hmult[0] = 0.0;
@@ -2039,6 +2060,10 @@ int sssp_debug(const DevGraph& g, Workspace* ws, const Params& prm, int32_t s, i
     if (depth[v] < 0) { sigma[v] = 0; delta[v] = 0.0; }
   // the source's delta is not computed by the sweep (depths D-2..1 only)
   delta[s] = 0.0;
+  g.to_caller(depth);
+  g.to_caller(sigma);
+  g.to_caller(delta);
+  (void)s_caller;
   int rc = sweep_cleanup(g, ws, prm);
   if (ws->hctl->sigma_ovf) return set_error(BCMI_ERR_OVERFLOW, "a shortest-path count exceeded 2^64-1");
   return rc;
@@ -2059,7 +2084,7 @@ int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32
   }
   hipLaunchKernelGGL(k_fill_f64, grid_for(64 * n), 256, 0, st, ws->dbg_delta, 0.0, 64 * n);
   for (int q = 0; q < 64; q++) {
-    ws->hsrc[q] = q < k ? src[q] : 0;
+    ws->hsrc[q] = q < k ? g.to_int(src[q]) : 0;
     ws->hmult[q] = 0.0;
   }
   HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, 64 * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -2095,6 +2120,11 @@ int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32
           delta[(int64_t)q * n + v] = d >= 1 ? dd[(size_t)v * 64 + q] : 0.0;
         }
     }
+  for (int q = 0; q < k; q++) {
+    g.to_caller(depth + (int64_t)q * n);
+    g.to_caller(sigma + (int64_t)q * n);
+    g.to_caller(delta + (int64_t)q * n);
+  }
   rc = sweep_cleanup(g, ws, prm);
   if (ws->hctl->sigma_ovf) return set_error(BCMI_ERR_OVERFLOW, "a shortest-path count exceeded 2^64-1");
   return rc;

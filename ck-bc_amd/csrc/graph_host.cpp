@@ -363,6 +363,29 @@ int gen_block_chain(int64_t nblocks, int block, double avgdeg, uint64_t seed, bc
   return BCMI_OK;
 }
 
+// Relabels vertices by descending degree (ties by id): new id i holds old
+// vertex n2o[i].  The adjacency of each vertex keeps its original order
+// (entries renamed, not re-sorted), so per-vertex sums keep their order.
+void relabel_by_degree(int64_t n, const int64_t* off, const int32_t* adj, std::vector<int32_t>& n2o,
+                       std::vector<int32_t>& o2n, std::vector<int64_t>& noff, std::vector<int32_t>& nadj) {
+  n2o.resize(n);
+  o2n.resize(n);
+  std::iota(n2o.begin(), n2o.end(), 0);
+  std::stable_sort(n2o.begin(), n2o.end(), [&](int32_t a, int32_t b) {
+    return off[a + 1] - off[a] > off[b + 1] - off[b];
+  });
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) o2n[n2o[i]] = (int32_t)i;
+  noff.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; i++) noff[i + 1] = noff[i] + (off[n2o[i] + 1] - off[n2o[i]]);
+  nadj.resize(noff[n]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t i = 0; i < n; i++) {
+    int64_t s = off[n2o[i]], d = noff[i], k = noff[i + 1] - noff[i];
+    for (int64_t e = 0; e < k; e++) nadj[d + e] = o2n[adj[s + e]];
+  }
+}
+
 // Uniform sample of k distinct vertices with degree > 0 (isolated sources
 // contribute nothing and the reference skips them, bc2d.c:1026-1035),
 // returned in draw order.

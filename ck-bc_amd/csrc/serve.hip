@@ -177,6 +177,7 @@ static void worker(bcmi_server* s) {
                                     workspace_stream(sl->ws));
       if (e == hipSuccess) e = hipStreamSynchronize(workspace_stream(sl->ws));
       if (e != hipSuccess) { rc = BCMI_ERR_HIP; err = hipGetErrorString(e); }
+      g.to_caller(bc.data());
     }
     lk.lock();
     q->bc.swap(bc);

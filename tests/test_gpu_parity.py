@@ -219,3 +219,47 @@ def test_kernel_timer(engine):
     ms, launches = engine.kernel_time()
     assert launches > 0 and ms > 0
     engine.set_param("ktime", 0)
+
+
+def test_relabel_off_and_on_agree():
+    """Degree relabelling is internal: ids at the ABI are the caller's."""
+    g = bcmi.Graph.rmat(11, 16, seed=21)
+    src = g.sample_sources(100, seed=3)
+    ref = O.brandes(g.off, g.adj, src, nthreads=4)
+    for reorder in (0, 1):
+        e2 = bcmi.Engine(reorder=reorder)
+        e2.upload(g)
+        for mode in (bcmi.MODE_PLAIN, bcmi.MODE_BATCHED):
+            e2.reset()
+            e2.run(src, mode=mode)
+            assert bcmi.rel_err(e2.bc(), ref) <= BC_TOL, (reorder, mode)
+        d, sg, dl = e2.sssp(int(src[0]))
+        od, osg, odl, _ = O.sssp(g.off, g.adj, int(src[0]))
+        assert np.array_equal(d, od) and np.array_equal(sg, osg)
+        dm, sgm, dlm = e2.msbfs(src[:8])
+        for q in range(8):
+            od, osg, odl, _ = O.sssp(g.off, g.adj, int(src[q]))
+            odl[src[q]] = 0.0
+            assert np.array_equal(dm[q], od) and np.array_equal(sgm[q], osg)
+            assert bcmi.rel_err(dlm[q], odl) <= 1e-12
+        e2.close()
+
+
+@pytest.mark.parametrize("reorder", [0, 1])
+def test_batched_delta_bit_exact_when_no_vertex_is_split(reorder):
+    """With every degree <= the record size, each lane sums its successor
+    terms in adjacency order with the oracle's operations (no FP contraction),
+    so delta matches the oracle bit for bit."""
+    e2 = bcmi.Engine(reorder=reorder)
+    for g in (bcmi.Graph.block_chain(30, 128, 8.0, seed=4), bcmi.Graph.from_edges(*G.random_graph(3000, 12000, 5))):
+        if g.degrees().max() > 256:
+            continue
+        e2.upload(g)
+        src = g.sample_sources(64, seed=9)
+        dm, sgm, dlm = e2.msbfs(src)
+        for q in range(64):
+            od, osg, odl, _ = O.sssp(g.off, g.adj, int(src[q]))
+            odl[src[q]] = 0.0
+            assert np.array_equal(dm[q], od) and np.array_equal(sgm[q], osg)
+            assert np.array_equal(dlm[q], odl), q
+    e2.close()
