@@ -45,6 +45,71 @@ def per_source_bytes_batched(st_n, st_m, st_a, uni_n, uni_m):
     return 24.0 * uni_m + 32.0 * uni_n + float(np.sum(24.0 * st_a + 32.0 * st_n))
 
 
+def serve_bench(args, log):
+    """Served BC queries: a pool of graphs of uneven size (R-MAT scales 12-20,
+    edge factor 8-32, and bridged-block chains of 256-vertex Erdos-Renyi
+    blocks), Poisson arrivals, each query = (graph, 16..1024 sampled sources).
+    Returns p50/p99 latency (arrival to completion) and aggregate GTEPS."""
+    import threading
+    import bcmi
+    rng = np.random.default_rng(args.seed + 7)
+    graphs = []
+    for i in range(args.serve_graphs):
+        if i % 4 == 3:
+            nb = int(rng.integers(64, args.serve_max_blocks + 1))
+            graphs.append(bcmi.Graph.block_chain(nb, 256, 8.0, seed=args.seed + 100 + i))
+        else:
+            sc = int(rng.integers(12, 21))
+            ef = int(rng.integers(8, 33))
+            graphs.append(bcmi.Graph.rmat(sc, ef, seed=args.seed + 100 + i))
+    srv = bcmi.Server(args.serve_budget_gb << 30, args.serve_slots)
+    ids = [srv.add_graph(g) for g in graphs]
+    nq = args.serve_queries
+    gsel = rng.integers(0, len(graphs), nq)
+    ks = rng.integers(16, 1025, nq)
+    gaps = rng.exponential(1.0 / args.serve_rate, nq)
+    qsrc = [graphs[gi].sample_sources(min(int(k), int((graphs[gi].degrees() > 0).sum())), seed=args.seed + 1000 + i)
+            for i, (gi, k) in enumerate(zip(gsel, ks))]
+    # warm each graph's workspace once (first use captures the level loops)
+    for gid, g in zip(ids, graphs):
+        q = srv.submit(gid, g.sample_sources(1, seed=1))
+        srv.wait(q, g.n, want_bc=False)
+    qids = [None] * nq
+    results = [None] * nq
+
+    def waiter():
+        for i in range(nq):
+            while qids[i] is None:
+                time.sleep(0.0005)
+            results[i] = srv.wait(qids[i], graphs[gsel[i]].n, want_bc=False)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    for i in range(nq):
+        t_next += gaps[i]
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        qids[i] = srv.submit(ids[gsel[i]], qsrc[i])
+    th.join()
+    wall = time.perf_counter() - t0
+    lat = np.array([r[2].finish_ms - r[2].arrive_ms for r in results])
+    ok = all(r[0] == 0 for r in results)
+    te = sum(r[2].te for r in results)
+    first = min(r[2].arrive_ms for r in results)
+    last = max(r[2].finish_ms for r in results)
+    st = srv.stats()
+    srv.close()
+    out = {"queries": nq, "graphs": len(graphs), "rate_qps": args.serve_rate, "slots": args.serve_slots,
+           "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+           "gteps": round(te / ((last - first) / 1e3) / 1e9, 2), "all_ok": ok,
+           "peak_slots": st["peak_slots"], "peak_gb": round(st["peak_bytes"] / 2**30, 2), "wall_s": round(wall, 1)}
+    log(f"serving: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +124,12 @@ def main():
     ap.add_argument("--seed", type=int, default=20250101)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="back", choices=["back", "sigma", "td", "bu"])
+    ap.add_argument("--serve-queries", type=int, default=48, help="0 skips the serving run")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (queries/s)")
+    ap.add_argument("--serve-graphs", type=int, default=32)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-budget-gb", type=int, default=96)
+    ap.add_argument("--serve-max-blocks", type=int, default=2048)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (PMC byte calibration)")
     args = ap.parse_args()
@@ -160,6 +231,8 @@ def main():
                          f"{t_cpu:.1f}s, OpenMP over sources"}
         log(f"cpu oracle: {cpu}")
 
+    serving = serve_bench(args, log) if args.serve_queries > 0 else None
+
     out = {
         "metric": "BC GTEPS on R-MAT scale-22 (batched vs plain, 1 GPU) + p50/p99 query latency",
         "value": round(gteps, 3),
@@ -185,6 +258,7 @@ def main():
                      "traffic": None, "kernel": f"k_ms_{args.kernel}",
                      "avg_launch_us": round(k_ms / max(k_launch, 1) * 1e3, 2)},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     e.close()

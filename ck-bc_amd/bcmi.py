@@ -50,7 +50,8 @@ class RunStats(ctypes.Structure):
     _fields_ = [("sources_run", ctypes.c_int64), ("batches", ctypes.c_int64), ("te", ctypes.c_double),
                 ("bytes", ctypes.c_double), ("gpu_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("max_levels", ctypes.c_int64), ("sum_n", ctypes.c_int64), ("sum_m", ctypes.c_int64),
-                ("sum_a", ctypes.c_int64)]
+                ("sum_a", ctypes.c_int64), ("sum_pairs", ctypes.c_int64), ("sum_union", ctypes.c_int64),
+                ("scaled_batches", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -306,3 +307,42 @@ def rel_err(a, b):
     if not nz.any():
         return 0.0
     return float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz])))
+
+
+# ---------------------------------------------------------------- server --
+class Server:
+    """Query server: device-resident graph pool, continuous batching."""
+
+    def __init__(self, hbm_budget, max_slots=8, device=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().bcmi_server_create(device, int(hbm_budget), int(max_slots), ctypes.byref(self._h)))
+        self.sizes = []
+
+    def add_graph(self, g):
+        gid = ctypes.c_int32()
+        _check(lib().bcmi_server_add_graph(self._h, g.n, _p(g.off), _p(g.adj), ctypes.byref(gid)))
+        self.sizes.append(g.n)
+        return gid.value
+
+    def submit(self, gid, sources):
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        q = ctypes.c_int64()
+        _check(lib().bcmi_server_submit(self._h, int(gid), _p(src), src.shape[0], ctypes.byref(q)))
+        return q.value
+
+    def wait(self, qid, n, want_bc=True):
+        """Returns (status, bc or None, QueryResult); status != 0 is an error code."""
+        out = np.zeros(n, np.float64) if want_bc else None
+        res = QueryResult()
+        rc = lib().bcmi_server_wait(self._h, int(qid), _p(out) if want_bc else None, ctypes.byref(res))
+        return rc, out, res
+
+    def stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().bcmi_server_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(completed=a.value, peak_slots=b.value, peak_bytes=c.value)
+
+    def close(self):
+        if self._h:
+            lib().bcmi_server_destroy(self._h)
+            self._h = ctypes.c_void_p()

@@ -118,6 +118,21 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "width") {
     if (v < 1 || v > 64) return set_error(BCMI_ERR_ARG, "width must be in [1,64]");
     p.width = (int)v;
+  } else if (k == "sgrid") {
+    if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "sgrid must be in [1,65536]");
+    p.sgrid = (int)v;
+  } else if (k == "sparse_verts") {
+    if (v < 0) return set_error(BCMI_ERR_ARG, "sparse_verts must be >= 0");
+    p.sparse_verts = v;
+  } else if (k == "sigma") {
+    if (v < 0 || v > 2) return set_error(BCMI_ERR_ARG, "sigma must be 0 (exact), 1 (scaled) or 2 (auto)");
+    p.sigma = (int)v;
+  } else if (k == "kflags") {
+    if (v < 0 || v > 3) return set_error(BCMI_ERR_ARG, "kflags must be 0..3");
+    p.kflags = (int)v;
+  } else if (k == "batching") {
+    if (v < 0 || v > 3) return set_error(BCMI_ERR_ARG, "batching must be 0..3");
+    p.batching = (int)v;
   } else if (k == "reorder") {
     if (c->have_graph) return set_error(BCMI_ERR_STATE, "reorder must be set before uploading a graph");
     p.reorder = v ? 1 : 0;
@@ -147,6 +162,11 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "ktime") *v = p.ktime;
   else if (k == "trace") *v = p.trace;
   else if (k == "reorder") *v = p.reorder;
+  else if (k == "batching") *v = p.batching;
+  else if (k == "kflags") *v = p.kflags;
+  else if (k == "sigma") *v = p.sigma;
+  else if (k == "sgrid") *v = p.sgrid;
+  else if (k == "sparse_verts") *v = p.sparse_verts;
   else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
   return BCMI_OK;
 }
@@ -249,6 +269,9 @@ int bcmi_bc_run(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int mode, int
     st->sum_n = R.sum_n;
     st->sum_m = R.sum_m;
     st->sum_a = R.sum_a;
+    st->sum_pairs = R.pairs;
+    st->scaled_batches = R.scaled_batches;
+    for (int64_t u : R.uni_n) st->sum_union += u;
   }
   return rc;
 }

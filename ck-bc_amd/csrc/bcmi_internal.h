@@ -21,12 +21,20 @@ void relabel_by_degree(int64_t n, const int64_t* off, const int32_t* adj, std::v
 // Tunable launch and algorithm parameters.
 struct Params {
   int block = 256;          // threads per block (multiple of 64)
-  int grid = 2048;          // persistent grid (blocks)
+  int grid = 4096;          // blocks per expansion kernel (about twice what is resident)
   int chunk = 256;          // max edges per work item
   int64_t alpha_plain = 1000;   // x1000; bottom-up when frontier arcs > alpha * unvisited arcs
   int64_t alpha_batched = 1000; // x1000
   int steps = 16;           // levels per captured graph
+  int sgrid = 128;          // blocks per kernel in the small-grid (sparse-level) graphs
+  int64_t sparse_verts = 4096;  // levels below this many vertices replay the small-grid graphs
   int width = 64;           // sources per batch
+  int sigma = 2;            // path counts: 0 exact uint64 (error on overflow), 1 scaled FP64,
+                            // 2 exact, batches that overflow are recomputed scaled
+  int kflags = 0;           // kernel variants: 1 gate mask probes with level bitmaps,
+                            // 2 non-temporal adjacency loads
+  int batching = 2;         // source order for batches: 0 as given, 1 hub-depth signature,
+                            // 2 highest-degree neighbour then signature
   int reorder = 1;          // relabel vertices by descending degree at upload (locality)
   int trace = 0;            // diagnostic: per-level statistics on stderr (use with steps = 1)
   int ktime = 0;            // kernel timed with HIP events inside the level loop:

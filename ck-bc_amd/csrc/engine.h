@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <vector>
 
@@ -25,6 +26,8 @@ struct Ctl {
   int32_t sigma_ovf;  // a path count wrapped past 2^64-1
   int32_t src_label;  // component of the plain-sweep source
   int32_t width;      // sources in the batch
+  int32_t scaled;     // batched: path counts held as FP64 scaled per level
+  int32_t pad_;
   int64_t rec_count;  // records appended so far
   int64_t arcs_next;  // degree sum of vertices appended during this step
   int64_t verts_next;
@@ -70,6 +73,7 @@ struct DevGraph {
   std::vector<int64_t> h_off;    // traversed offsets on the host (degrees)
   std::vector<int32_t> h_round;  // removal round per vertex on the host
   std::vector<uint32_t> h_sig;   // batching key: BFS depths from the four largest hubs
+  std::vector<int32_t> h_topnb;  // batching key: highest-degree neighbour
   // Vertices are relabelled by descending degree on upload (hubs' rows and
   // masks then share cache lines); the caller's ids are mapped at the ABI.
   bool relabeled = false;
@@ -83,6 +87,9 @@ struct DevGraph {
     std::vector<T> t(a, a + n);
     for (int64_t v = 0; v < n; v++) a[v] = t[h_o2n[v]];
   }
+  // set once a batched sweep's exact path counts wrapped (auto mode then
+  // starts later batches in scaled FP64)
+  mutable std::atomic<bool> sigma_overflowed{false};
   size_t bytes = 0;
   void release();
 };
@@ -100,6 +107,8 @@ struct RunResult {
   int64_t sweeps = 0, launches = 0, max_levels = 0;
   int64_t sum_n = 0, sum_m = 0, sum_a = 0;
   int64_t n_td = 0, n_bu = 0;
+  int64_t pairs = 0;                  // batched: (vertex, depth) pairs
+  int64_t scaled_batches = 0;         // batches recomputed with scaled FP64 path counts
   std::vector<int64_t> uni_n, uni_m;  // batched: union reach per batch
   double te = 0, bytes = 0, gpu_ms = 0, wall_ms = 0;
   double ktime_ms = 0;      // summed duration of the timed kernel (Params::ktime)

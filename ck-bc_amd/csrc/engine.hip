@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -121,6 +122,16 @@ __device__ __forceinline__ double ld_coherent(const double* p) {
 __device__ __forceinline__ u64 ld_coherent_u64(const u64* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Adjacency lists are streamed once per level: keep them out of L2 so the
+// masks and rows gathered at random stay resident.
+#define KF_GATE 1  // gate mask probes with the level bitmaps
+#define KF_NT 2    // non-temporal adjacency loads
+// One bit per vertex: "this vertex has a non-zero mask on this level".
+// Three bitmaps mirror the rotating level masks; at 4M vertices each is
+// 512 KB, small enough to stay in L2, and gates the 8-byte mask probes.
+__device__ __forceinline__ bool lb_test(const unsigned* b, int32_t v) { return (b[v >> 5] >> (v & 31)) & 1u; }
+__device__ __forceinline__ void lb_set(unsigned* b, int32_t v) { atomicOr(&b[v >> 5], 1u << (v & 31)); }
+__device__ __forceinline__ void lb_clr(unsigned* b, int32_t v) { atomicAnd(&b[v >> 5], ~(1u << (v & 31))); }
 __device__ __forceinline__ int nchunks(int64_t deg, int chunk) {
   return deg <= chunk ? 1 : (int)((deg + chunk - 1) / chunk);
 }
@@ -168,7 +179,12 @@ struct GA {
   const uint64_t* w;
   int chunk;
   int64_t n;
+  int flags;  // KF_*
 };
+
+__device__ __forceinline__ int32_t adj_at(const GA& g, const int32_t* a, int64_t e) {
+  return (g.flags & KF_NT) ? __builtin_nontemporal_load(a + e) : a[e];
+}
 
 struct PA {
   Ctl* ctl;
@@ -191,6 +207,8 @@ struct BA {
   Ctl* ctl;
   u64* seen;
   u64* lm;  // 3 arrays of n
+  unsigned* lb;  // 3 bitmaps of nw words
+  int64_t nw;
   u64* sigma;
   double* coef;
   int32_t* done;
@@ -209,6 +227,9 @@ struct BA {
   double* dbg_delta;
   int64_t alpha;
   int64_t n;
+  double* lscale;      // [level][64] scale factor m of each level (scaled path counts)
+  int64_t lscale_cap;  // levels it can hold
+  u64* lvmax;          // [64] running max of the level being built (double bits)
 };
 
 // Work-list appends are staged per wave in LDS and flushed with one global
@@ -324,6 +345,7 @@ __global__ void k_plain_init(GA g, PA p) {
   c->sum_a = 0; c->n_td = 0; c->n_bu = 0;
   c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 1; c->uni_arcs = deg;
   c->sigma_ovf = 0;
+  c->scaled = 0;
   p.rec_lo[0] = 0;
   for (int k = 0; k < nch && k < p.cap; k++) { p.rec_v[k] = s; p.rec_j[k] = k; }
   if (nch > p.cap) c->overflow = 1;
@@ -352,14 +374,14 @@ __device__ __forceinline__ bool decide_common(Ctl* c, int64_t* rec_lo, int64_t a
   return true;
 }
 
-__global__ void k_plain_decide(GA g, PA p) {
+__global__ void k_plain_decide(GA g, PA p, int allow_bu) {
   Ctl* c = p.ctl;
   if (!decide_common(c, p.rec_lo, p.alpha)) return;
   // Top-down reads the frontier's arcs; bottom-up reads the arcs of every
   // unvisited vertex of the component (no early exit: BC needs all parents).
   c->arcs_seen += c->arcs_cur;
   int64_t unvisited = c->arcs_reach - c->arcs_seen;
-  bool bu = !g.directed && (double)c->arcs_cur * 1000.0 > (double)p.alpha * (double)unvisited;
+  bool bu = allow_bu && !g.directed && (double)c->arcs_cur * 1000.0 > (double)p.alpha * (double)unvisited;
   c->mode = bu ? 1 : 0;
   if (bu) c->n_bu++; else c->n_td++;
 }
@@ -386,7 +408,7 @@ __device__ __forceinline__ void k_plain_td_body(GA g, PA p) {
     for (int64_t eb = e0; eb < e1; eb += 64) {
       int64_t e = eb + lane;
       bool valid = e < e1;
-      int32_t v = valid ? g.adj[e] : 0;
+      int32_t v = valid ? adj_at(g, g.adj, e) : 0;
       bool won = false;
       int64_t dg = 0;
       if (valid) {
@@ -447,7 +469,7 @@ __device__ __forceinline__ void k_plain_bu_body(GA g, PA p) {
     for (int64_t eb = e0; eb < e1; eb += 64) {
       int64_t e = eb + lane;
       if (e < e1) {
-        int32_t u = g.adj[e];
+        int32_t u = adj_at(g, g.adj, e);
         if (p.depth[u] == d) {
           u64 su = p.sigma[u];
           u64 s = acc + su;
@@ -486,7 +508,9 @@ __global__ void __launch_bounds__(256) k_plain_bu(GA g, PA p, KT* kt) {
 // so delta = w(v)) up to depth 1; the source's own delta is never needed.
 __global__ void k_bdecide(Ctl* c) {
   if (!c->fdone || c->bdone) return;
-  if (c->overflow) {
+  // A work-list overflow is retried; an exact path count that wrapped makes
+  // the sweep invalid, so its backward pass (and BC update) is skipped.
+  if (c->overflow || (c->sigma_ovf && !c->scaled)) {
     c->bdone = 1;
     return;
   }
@@ -524,7 +548,7 @@ __device__ __forceinline__ void k_plain_back_body(GA g, PA p) {
     for (int64_t eb = e0; eb < e1; eb += 64) {
       int64_t e = eb + lane;
       if (e < e1) {
-        int32_t z = g.adj[e];
+        int32_t z = adj_at(g, g.adj, e);
         if (p.depth[z] == d + 1) acc = acc + p.coef[z] * sv;
       }
     }
@@ -587,7 +611,7 @@ __global__ void k_plain_cleanup(PA p) {
 // seen (sources that reached v so far) and three rotating level arrays lm[k]
 // with lm[d%3] = mask of sources for which v has depth d, kept exact for the
 // levels a kernel reads.
-__global__ void k_ms_init(GA g, BA b, int k) {
+__global__ void k_ms_init(GA g, BA b, int k, int scaled) {
   Ctl* c = b.ctl;
   const int i = threadIdx.x;  // one block of 64 threads
   u64* lm0 = b.lm;
@@ -599,16 +623,21 @@ __global__ void k_ms_init(GA g, BA b, int k) {
     c->arcs_next = 0; c->verts_next = 0; c->arcs_cur = 0; c->verts_cur = 0;
     c->arcs_seen = 0; c->sum_a = 0; c->n_td = 0; c->n_bu = 0;
     c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 0; c->uni_arcs = 0;
-    c->sigma_ovf = 0; c->width = k;
+    c->sigma_ovf = 0; c->width = k; c->scaled = scaled;
     b.rec_lo[0] = 0;
   }
   b.st_n[i] = 0; b.st_m[i] = 0; b.st_a[i] = 0;
+  if (scaled) {
+    b.lvmax[i] = 0;
+    b.lscale[i] = 1.0;
+  }
   int32_t s = i < k ? b.srcbuf[i] : 0;
   if (i < k) {
     atomicOr(&lm0[s], 1ull << i);
+    lb_set(b.lb, s);
     atomicOr(&b.seen[s], 1ull << i);
     atomicOr(&b.compmask[g.label[s]], 1ull << i);
-    b.sigma[(int64_t)s * 64 + i] = 1;
+    b.sigma[(int64_t)s * 64 + i] = scaled ? (u64)__double_as_longlong(1.0) : 1ull;
   }
   __syncthreads();
   if (i < k) {
@@ -656,11 +685,27 @@ __global__ void k_ms_init(GA g, BA b, int k) {
   }
 }
 
-__global__ void k_ms_decide(GA g, BA b) {
+__global__ void k_ms_decide(GA g, BA b, int allow_bu) {
   Ctl* c = b.ctl;
   if (!decide_common(c, b.rec_lo, b.alpha)) return;
+  if (c->scaled) {
+    // Scaled path counts: level d holds raw(v) = sigma(v)/S(d-1); its factor
+    // m(d) (the largest raw value, 1 if none) gives S(d) = S(d-1) m(d).
+    const int d = c->level;
+    if (d >= b.lscale_cap) {
+      c->overflow = 2;
+      c->fdone = 1;
+      c->last = d - 1;
+      return;
+    }
+    for (int q = 0; q < 64; q++) {
+      double mx = __longlong_as_double((long long)b.lvmax[q]);
+      b.lscale[(int64_t)d * 64 + q] = mx > 0.0 ? mx : 1.0;
+      b.lvmax[q] = 0;
+    }
+  }
   int64_t incomplete = c->arcs_reach - c->arcs_seen;
-  bool bu = !g.directed && (double)c->arcs_cur * 1000.0 > (double)b.alpha * (double)incomplete;
+  bool bu = allow_bu && !g.directed && (double)c->arcs_cur * 1000.0 > (double)b.alpha * (double)incomplete;
   c->mode = bu ? 1 : 0;
   if (bu) c->n_bu++; else c->n_td++;
 }
@@ -688,6 +733,7 @@ __device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
   if (c->fdone || c->mode != 0) return;
   const int d = c->level;
   u64* nxt = b.lm + (int64_t)((d + 1) % 3) * b.n;
+  unsigned* nlb = b.lb + (int64_t)((d + 1) % 3) * b.nw;
   const int64_t lo = b.rec_lo[d], hi = b.rec_lo[d + 1];
   const int lane = lane_id();
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -702,7 +748,7 @@ __device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
     for (int64_t eb = e0; eb < e1; eb += 64) {
       int64_t e = eb + lane;
       bool valid = e < e1;
-      int32_t v = valid ? g.adj[e] : 0;
+      int32_t v = valid ? adj_at(g, g.adj, e) : 0;
       bool won = false;
       int64_t dg = 0;
       if (valid) {
@@ -710,7 +756,10 @@ __device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
         if (bits && (nxt[v] & bits) != bits) {
           u64 old = atomicOr(&nxt[v], bits);
           won = (old == 0);
-          if (won) dg = g.off[v + 1] - g.off[v];
+          if (won) {
+            dg = g.off[v + 1] - g.off[v];
+            lb_set(nlb, v);
+          }
         }
       }
       app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
@@ -734,6 +783,8 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   const int d = c->level;
   const u64* cur = b.lm + (int64_t)(d % 3) * b.n;
   u64* nxt = b.lm + (int64_t)((d + 1) % 3) * b.n;
+  const unsigned* clb = b.lb + (int64_t)(d % 3) * b.nw;
+  unsigned* nlb = b.lb + (int64_t)((d + 1) % 3) * b.nw;
   const int lane = lane_id();
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
@@ -758,7 +809,10 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       if ((nxt[v] & acc) != acc) {
         u64 old = atomicOr(&nxt[v], acc);
         won = (old == 0);
-        if (won) dg = g.off[v + 1] - g.off[v];
+        if (won) {
+          dg = g.off[v + 1] - g.off[v];
+          lb_set(nlb, v);
+        }
       }
     }
     app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
@@ -783,6 +837,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   const int d = c->level;
   const u64* cur = b.lm + (int64_t)(d % 3) * b.n;
   const u64* nxt = b.lm + (int64_t)((d + 1) % 3) * b.n;
+  const unsigned* clb = b.lb + (int64_t)(d % 3) * b.nw;
   const int64_t lo = b.rec_lo[d + 1];
   const int64_t hi = min(c->rec_count, b.cap);
   const int lane = lane_id();
@@ -791,6 +846,9 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   u64 cn = 0, cm = 0, ca = 0;
   int ovf = 0;
   long long done_arcs = 0, uv = 0, ua = 0;
+  const bool scaled = c->scaled;
+  const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
+  double mx = 0.0;
   for (int64_t r = lo + gw; r < hi; r += tw) {
     int32_t v = b.rec_v[r];
     int j = b.rec_j[r];
@@ -808,13 +866,14 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
     }
     const bool mine = (N >> lane) & 1;
     u64 acc = 0;
+    double accf = 0.0;
     for (int64_t eb = e0; eb < e1; eb += 64) {
       int64_t e = eb + lane;
       int32_t u = 0;
       u64 F = 0;
       if (e < e1) {
-        u = g.tadj[e];
-        F = cur[u] & N;
+        u = adj_at(g, g.tadj, e);
+        if (!(g.flags & KF_GATE) || lb_test(clb, u)) F = cur[u] & N;
       }
       u64 P = __ballot(F != 0);
       // Parents four at a time: the four row loads are issued before any is
@@ -837,16 +896,28 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
           x[t] = use ? b.sigma[(int64_t)uq * 64 + lane] : 0;
           ca += use;
         }
+        if (scaled) {
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-          u64 s = acc + x[t];
-          if (s < acc) ovf = 1;
-          acc = s;
+          for (int t = 0; t < 4; t++) accf = accf + __longlong_as_double((long long)x[t]);
+        } else {
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            u64 s = acc + x[t];
+            if (s < acc) ovf = 1;
+            acc = s;
+          }
         }
       }
     }
     u64* row = b.sigma + (int64_t)v * 64 + lane;
-    if (nch == 1) {
+    if (scaled) {
+      double raw = accf * inv_m;
+      if (mine) {
+        if (nch == 1) *row = (u64)__double_as_longlong(raw);
+        else atomicAdd((double*)row, raw);
+        mx = fmax(mx, raw);
+      }
+    } else if (nch == 1) {
       if (mine) *row = acc;
     } else if (mine) {
       u64 o = atomicAdd(row, acc);
@@ -869,10 +940,14 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   // clear level d-1 (array (d+2)%3)
   if (d >= 1) {
     u64* old = b.lm + (int64_t)((d + 2) % 3) * b.n;
+    unsigned* olb = b.lb + (int64_t)((d + 2) % 3) * b.nw;
     const int64_t plo = b.rec_lo[d - 1], phi = b.rec_lo[d];
     for (int64_t r = plo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < phi;
          r += (int64_t)gridDim.x * blockDim.x)
-      if (b.rec_j[r] == 0) old[b.rec_v[r]] = 0;
+      if (b.rec_j[r] == 0) {
+        old[b.rec_v[r]] = 0;
+        lb_clr(olb, b.rec_v[r]);
+      }
   }
   if (cn) atomicAdd(&b.st_n[lane], cn);
   if (cm) atomicAdd(&b.st_m[lane], cm);
@@ -881,6 +956,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   if (uv) atomicAdd((u64*)&c->uni_verts, (u64)uv);
   if (ua) atomicAdd((u64*)&c->uni_arcs, (u64)ua);
   if (ovf) c->sigma_ovf = 1;
+  if (scaled && mx > 0.0) atomicMax(&b.lvmax[lane], (u64)__double_as_longlong(mx));
 }
 
 __global__ void __launch_bounds__(256) k_ms_sigma(GA g, BA b, KT* kt) {
@@ -901,11 +977,17 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   const int d = c->blevel;
   const u64* lmn = b.lm + (int64_t)((d + 1) % 3) * b.n;
   u64* lmd = b.lm + (int64_t)(d % 3) * b.n;
+  const unsigned* nlb = b.lb + (int64_t)((d + 1) % 3) * b.nw;
+  unsigned* dlb = b.lb + (int64_t)(d % 3) * b.nw;
   const int64_t lo = b.rec_lo[d], hi = b.rec_lo[d + 1];
   const int lane = lane_id();
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   const double mult = b.multbuf[lane];
+  // Scaled path counts: sigma_v/sigma_w = raw(v) / (raw(w) m(d)) for v on
+  // level d and w on level d+1, so the terms use raw(v)/m(d) and coef keeps raw.
+  const bool scaled = c->scaled;
+  const double inv_md = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   for (int64_t r = lo + gw; r < hi; r += tw) {
     int32_t v = b.rec_v[r];
     int j = b.rec_j[r];
@@ -916,15 +998,17 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
     int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
     const bool mine = (L >> lane) & 1;
     const double wv = g.w ? (double)g.w[v] : 0.0;
-    const double sv = mine ? (double)b.sigma[(int64_t)v * 64 + lane] : 1.0;
+    const u64 sbits = mine ? b.sigma[(int64_t)v * 64 + lane] : 1ull;
+    const double svr = !mine ? 1.0 : scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
+    const double sv = scaled ? svr * inv_md : svr;
     double acc = (nch == 1) ? wv : 0.0;
     for (int64_t eb = e0; eb < e1; eb += 64) {
       int64_t e = eb + lane;
       int32_t z = 0;
       u64 M = 0;
       if (e < e1) {
-        z = g.adj[e];
-        M = lmn[z] & L;
+        z = adj_at(g, g.adj, e);
+        if (!(g.flags & KF_GATE) || lb_test(nlb, z)) M = lmn[z] & L;
       }
       u64 P = __ballot(M != 0);
       // Successors four at a time (loads first, then the sums in adjacency
@@ -958,22 +1042,29 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
     } else {
       double contrib = 0.0;
       if (mine) {
-        *crow = (acc + 1.0) / sv;
+        *crow = (acc + 1.0) / svr;
         contrib = mult * acc;
         if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = acc;
       }
       contrib = wave_sum_d(contrib);
       if (lane == 0) b.bc[v] += contrib;
     }
-    if (j == 0 && lane == 0) lmd[v] = L;
+    if (j == 0 && lane == 0) {
+      lmd[v] = L;
+      lb_set(dlb, v);
+    }
   }
   {
     u64* old = b.lm + (int64_t)((d + 2) % 3) * b.n;
+    unsigned* olb = b.lb + (int64_t)((d + 2) % 3) * b.nw;
     const int64_t plo = b.rec_lo[d + 2], phi = b.rec_lo[d + 3];
     if (d + 2 <= c->last)
       for (int64_t r = plo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < phi;
            r += (int64_t)gridDim.x * blockDim.x)
-        if (b.rec_j[r] == 0) old[b.rec_v[r]] = 0;
+        if (b.rec_j[r] == 0) {
+          old[b.rec_v[r]] = 0;
+          lb_clr(olb, b.rec_v[r]);
+        }
   }
 }
 
@@ -1005,7 +1096,9 @@ __global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b) {
     if (mine) {
       double* crow = b.coef + (int64_t)v * 64 + lane;
       double dl = (g.w ? (double)g.w[v] : 0.0) + *crow;
-      *crow = (dl + 1.0) / (double)b.sigma[(int64_t)v * 64 + lane];
+      u64 sbits = b.sigma[(int64_t)v * 64 + lane];
+      double svr = c->scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
+      *crow = (dl + 1.0) / svr;
       contrib = mult * dl;
       if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = dl;
     }
@@ -1025,6 +1118,9 @@ __global__ void k_ms_cleanup(GA g, BA b) {
       b.lm[v] = 0;
       b.lm[b.n + v] = 0;
       b.lm[2 * b.n + v] = 0;
+      lb_clr(b.lb, v);
+      lb_clr(b.lb + b.nw, v);
+      lb_clr(b.lb + 2 * b.nw, v);
       b.compmask[g.label[v]] = 0;
     }
   }
@@ -1243,6 +1339,7 @@ void DevGraph::release() {
   fr(foff); fr(fadj); fr(label); fr(comp_size); fr(comp_arcs); fr(all_v); fr(all_j);
   fr(round); fr(parent); fr(w); fr(root); fr(order); fr(d_n2o);
   d_n2o = nullptr; relabeled = false; h_n2o.clear(); h_o2n.clear();
+  sigma_overflowed.store(false);
   off = foff = toff = nullptr; adj = fadj = tadj = nullptr; label = nullptr; comp_size = comp_arcs = nullptr;
   all_v = all_j = nullptr; round = parent = root = order = nullptr; w = nullptr;
   n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); h_sig.clear(); bytes = 0;
@@ -1435,6 +1532,10 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
   }
   g.h_off = hoff;
   hub_signature(n, off, adj, g.h_sig);
+  // first neighbour in descending-degree order (the smallest internal id once relabelled)
+  g.h_topnb.assign(n, INT32_MAX);
+  for (int64_t v = 0; v < n; v++)
+    for (int64_t e = off[v]; e < off[v + 1]; e++) g.h_topnb[v] = std::min(g.h_topnb[v], adj[e]);
   g.h_round.assign(n, 0);
   if (heuristic) {
     HIP_TRY(hipMemcpyAsync(g.h_round.data(), g.round, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
@@ -1504,12 +1605,17 @@ struct Workspace {
   // batched
   u64* seen = nullptr;
   u64* lm = nullptr;
+  unsigned* lb = nullptr;
+  int64_t nw = 0;
   u64* sigma = nullptr;
   double* coef = nullptr;
   u64* compmask = nullptr;
   u64* stats = nullptr;   // 3 x 64
   u64* hstats = nullptr;  // pinned
   double* dbg_delta = nullptr;
+  double* lscale = nullptr;  // scaled path counts: per-level factors
+  int64_t lscale_cap = 0;
+  u64* lvmax = nullptr;
   // per-run source terms (peeling)
   u64* cnt0 = nullptr;
   u64* cntS = nullptr;
@@ -1518,7 +1624,8 @@ struct Workspace {
   int32_t* srcdev = nullptr;
   int64_t srcdev_cap = 0;
   // captured level loops
-  hipGraphExec_t gfwd = nullptr, gbwd = nullptr;
+  hipGraphExec_t gfwd = nullptr, gbwd = nullptr;      // large grids, both directions
+  hipGraphExec_t gfwd_s = nullptr, gbwd_s = nullptr;  // small grids, top-down only (sparse levels)
   int cap_steps = 0;
   Params cap_prm;
   KT* kt = nullptr;             // device timer of the kernel selected by Params::ktime
@@ -1527,13 +1634,14 @@ struct Workspace {
   size_t bytes = 0;
 };
 
-static GA make_ga(const DevGraph& g) {
+static GA make_ga(const DevGraph& g, const Params& prm) {
   GA a;
   a.off = g.off; a.adj = g.adj; a.toff = g.toff; a.tadj = g.tadj; a.directed = g.directed;
   a.label = g.label; a.comp_arcs = g.comp_arcs;
   a.all_v = g.all_v; a.all_j = g.all_j; a.nall = g.nall;
   a.w = g.heuristic ? (const uint64_t*)g.w : nullptr;
   a.chunk = g.chunk; a.n = g.n;
+  a.flags = prm.kflags;
   return a;
 }
 static PA make_pa(Workspace* ws, const Params& prm) {
@@ -1545,11 +1653,12 @@ static PA make_pa(Workspace* ws, const Params& prm) {
 }
 static BA make_ba(Workspace* ws, const Params& prm) {
   BA b;
-  b.ctl = ws->ctl; b.seen = ws->seen; b.lm = ws->lm; b.sigma = ws->sigma; b.coef = ws->coef; b.done = ws->done;
+  b.ctl = ws->ctl; b.seen = ws->seen; b.lm = ws->lm; b.lb = ws->lb; b.nw = ws->nw; b.sigma = ws->sigma; b.coef = ws->coef; b.done = ws->done;
   b.rec_v = ws->rec_v; b.rec_j = ws->rec_j; b.rec_mask = ws->rec_mask; b.cap = ws->cap; b.rec_lo = ws->rec_lo;
   b.compmask = ws->compmask; b.srcbuf = ws->srcbuf; b.multbuf = ws->multbuf; b.bc = ws->bc;
   b.st_n = ws->stats; b.st_m = ws->stats + 64; b.st_a = ws->stats + 128; b.dbg_delta = ws->dbg_delta;
   b.alpha = prm.alpha_batched; b.n = ws->n;
+  b.lscale = ws->lscale; b.lscale_cap = ws->lscale_cap; b.lvmax = ws->lvmax;
   return b;
 }
 
@@ -1573,9 +1682,10 @@ size_t workspace_bytes(const DevGraph& g, int mode, int width, const Params& prm
 }
 
 static void free_graphs(Workspace* ws) {
-  if (ws->gfwd) (void)hipGraphExecDestroy(ws->gfwd);
-  if (ws->gbwd) (void)hipGraphExecDestroy(ws->gbwd);
-  ws->gfwd = ws->gbwd = nullptr;
+  for (hipGraphExec_t* x : {&ws->gfwd, &ws->gbwd, &ws->gfwd_s, &ws->gbwd_s}) {
+    if (*x) (void)hipGraphExecDestroy(*x);
+    *x = nullptr;
+  }
   ws->cap_graph = nullptr;
 }
 
@@ -1595,7 +1705,7 @@ void workspace_destroy(Workspace* ws) {
   if (ws->own_bc) fr(ws->bc);
   fr(ws->ctl); fr(ws->rec_v); fr(ws->rec_j); fr(ws->rec_mask); fr(ws->rec_lo); fr(ws->done);
   fr(ws->srcbuf); fr(ws->multbuf); fr(ws->depth); fr(ws->sigma1); fr(ws->delta1); fr(ws->coef1);
-  fr(ws->seen); fr(ws->lm); fr(ws->sigma); fr(ws->coef); fr(ws->compmask); fr(ws->stats); fr(ws->dbg_delta);
+  fr(ws->seen); fr(ws->lm); fr(ws->lb); fr(ws->lscale); fr(ws->lvmax); fr(ws->sigma); fr(ws->coef); fr(ws->compmask); fr(ws->stats); fr(ws->dbg_delta);
   fr(ws->cnt0); fr(ws->cntS); fr(ws->sa); fr(ws->srcK); fr(ws->srcdev);
   if (ws->hctl) (void)hipHostFree(ws->hctl);
   if (ws->hkt) (void)hipHostFree(ws->hkt);
@@ -1662,6 +1772,12 @@ static int ws_init(Workspace* ws, const DevGraph& g, int mode, int width, const 
   } else {
     TRY(dalloc(&ws->seen, n, acc));
     TRY(dalloc(&ws->lm, 3 * n, acc));
+    ws->lscale_cap = std::min<int64_t>(n + 2, 4096);
+    TRY(dalloc(&ws->lscale, (size_t)ws->lscale_cap * 64, acc));
+    TRY(dalloc(&ws->lvmax, 64, acc));
+    ws->nw = (n + 31) / 32;
+    TRY(dalloc(&ws->lb, 3 * ws->nw, acc));
+    hipLaunchKernelGGL(k_fill_i32, grid_for(3 * ws->nw), 256, 0, st, (int32_t*)ws->lb, 0, 3 * ws->nw);
     TRY(dalloc(&ws->sigma, 64 * n, acc));
     TRY(dalloc(&ws->coef, 64 * n, acc));
     TRY(dalloc(&ws->compmask, n, acc));
@@ -1712,7 +1828,7 @@ static double now_ms() {
 static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
   ALLOC_LOCK;
   free_graphs(ws);
-  GA ga = make_ga(g);
+  GA ga = make_ga(g, prm);
   hipStream_t st = ws->st;
   int B = prm.block, G = prm.grid;
   // kernel timing: which kernel gets the device timer
@@ -1720,32 +1836,38 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
   const bool plain = ws->mode == BCMI_MODE_PLAIN;
   KT* T = ws->kt;
   auto tk = [&](int id) -> KT* { return kt == id ? T : nullptr; };
-  for (int which = 0; which < 2; which++) {
+  // Four graphs: {large grid, small grid} x {forward, backward}.  The small
+  // one serves levels with few vertices (long chains): its launches are cheap
+  // and it never considers bottom-up (which scans every vertex).
+  for (int which = 0; which < 4; which++) {
+    const bool fwd = (which & 1) == 0;
+    const bool sparse = which >= 2;
+    const int Gk = sparse ? prm.sgrid : G;
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < prm.steps; s++) {
       if (plain) {
         PA pa = make_pa(ws, prm);
-        if (which == 0) {
-          hipLaunchKernelGGL(k_plain_decide, 1, 1, 0, st, ga, pa);
-          hipLaunchKernelGGL(k_plain_td, G, B, 0, st, ga, pa, tk(5));
-          hipLaunchKernelGGL(k_plain_bu, G, B, 0, st, ga, pa, tk(6));
+        if (fwd) {
+          hipLaunchKernelGGL(k_plain_decide, 1, 1, 0, st, ga, pa, sparse ? 0 : 1);
+          hipLaunchKernelGGL(k_plain_td, Gk, B, 0, st, ga, pa, tk(5));
+          if (!sparse) hipLaunchKernelGGL(k_plain_bu, Gk, B, 0, st, ga, pa, tk(6));
         } else {
           hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, pa.ctl);
-          hipLaunchKernelGGL(k_plain_back, G, B, 0, st, ga, pa, tk(7));
-          hipLaunchKernelGGL(k_plain_bfin, G, B, 0, st, ga, pa);
+          hipLaunchKernelGGL(k_plain_back, Gk, B, 0, st, ga, pa, tk(7));
+          hipLaunchKernelGGL(k_plain_bfin, Gk, B, 0, st, ga, pa);
         }
       } else {
         BA ba = make_ba(ws, prm);
-        if (which == 0) {
-          hipLaunchKernelGGL(k_ms_decide, 1, 1, 0, st, ga, ba);
-          hipLaunchKernelGGL(k_ms_td, G, B, 0, st, ga, ba, tk(3));
-          hipLaunchKernelGGL(k_ms_bu, G, B, 0, st, ga, ba, tk(4));
-          hipLaunchKernelGGL(k_ms_sigma, G, B, 0, st, ga, ba, tk(1));
+        if (fwd) {
+          hipLaunchKernelGGL(k_ms_decide, 1, 1, 0, st, ga, ba, sparse ? 0 : 1);
+          hipLaunchKernelGGL(k_ms_td, Gk, B, 0, st, ga, ba, tk(3));
+          if (!sparse) hipLaunchKernelGGL(k_ms_bu, Gk, B, 0, st, ga, ba, tk(4));
+          hipLaunchKernelGGL(k_ms_sigma, Gk, B, 0, st, ga, ba, tk(1));
         } else {
           hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, ba.ctl);
-          hipLaunchKernelGGL(k_ms_back, G, B, 0, st, ga, ba, tk(2));
-          hipLaunchKernelGGL(k_ms_bfin, G, B, 0, st, ga, ba);
+          hipLaunchKernelGGL(k_ms_back, Gk, B, 0, st, ga, ba, tk(2));
+          hipLaunchKernelGGL(k_ms_bfin, Gk, B, 0, st, ga, ba);
         }
       }
     }
@@ -1755,7 +1877,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
     e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return set_error(BCMI_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-    if (which == 0) ws->gfwd = ex; else ws->gbwd = ex;
+    (which == 0 ? ws->gfwd : which == 1 ? ws->gbwd : which == 2 ? ws->gfwd_s : ws->gbwd_s) = ex;
   }
   ws->cap_steps = prm.steps;
   ws->cap_prm = prm;
@@ -1765,15 +1887,16 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
 
 static bool same_launch(const Params& a, const Params& b) {
   return a.block == b.block && a.grid == b.grid && a.chunk == b.chunk && a.alpha_plain == b.alpha_plain &&
-         a.alpha_batched == b.alpha_batched && a.steps == b.steps && a.ktime == b.ktime;
+         a.alpha_batched == b.alpha_batched && a.steps == b.steps && a.ktime == b.ktime &&
+         a.kflags == b.kflags && a.sgrid == b.sgrid;
 }
 
 // Runs one sweep (plain: srcbuf[0]; batched: srcbuf[0..k)) whose inputs are
 // already in the workspace buffers.  keep != 0 skips the cleanup (debug
 // readback), the caller then calls sweep_cleanup.
-static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, bool keep, int* levels) {
+static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, bool keep, int* levels, int scaled = 0) {
   if (!ws->gfwd || ws->cap_graph != &g || !same_launch(ws->cap_prm, prm)) TRY(capture(g, ws, prm));
-  GA ga = make_ga(g);
+  GA ga = make_ga(g, prm);
   hipStream_t st = ws->st;
   for (int attempt = 0;; attempt++) {
     if (ws->mode == BCMI_MODE_PLAIN) {
@@ -1781,7 +1904,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       hipLaunchKernelGGL(k_plain_init, 1, 1, 0, st, ga, pa);
     } else {
       BA ba = make_ba(ws, prm);
-      hipLaunchKernelGGL(k_ms_init, 1, 64, 0, st, ga, ba, k);
+      hipLaunchKernelGGL(k_ms_init, 1, 64, 0, st, ga, ba, k, scaled);
     }
     if (prm.trace) {
       // Diagnostic: one graph launch at a time, level statistics on stderr.
@@ -1811,7 +1934,9 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
     HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     while (!ws->hctl->fdone && !ws->hctl->overflow) {
-      HIP_TRY(hipGraphLaunch(ws->gfwd, st));
+      // small levels (long chains) replay the small-grid graph
+      int64_t front = std::max<int64_t>(ws->hctl->verts_cur, ws->hctl->verts_next);
+      HIP_TRY(hipGraphLaunch(front < prm.sparse_verts ? ws->gfwd_s : ws->gfwd, st));
       HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
@@ -1825,19 +1950,40 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       } else {
         hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->seen, 0ull, n);
         hipLaunchKernelGGL(k_fill_u64, grid_for(3 * n), 256, 0, st, ws->lm, 0ull, 3 * n);
+        hipLaunchKernelGGL(k_fill_i32, grid_for(3 * ws->nw), 256, 0, st, (int32_t*)ws->lb, 0, 3 * ws->nw);
         hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->compmask, 0ull, n);
       }
       hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, ws->done, 0, n);
       HIP_TRY(hipStreamSynchronize(st));
-      if (attempt > 8) return set_error(BCMI_ERR_NOMEM, "work list keeps overflowing");
-      TRY(alloc_records(ws, ws->cap * 2));
+      if (attempt > 40) return set_error(BCMI_ERR_NOMEM, "work list keeps overflowing");
+      if (ws->hctl->overflow == 2) {  // more levels than the scale table holds
+        ALLOC_LOCK;
+        int64_t cap = std::min<int64_t>(ws->n + 2, ws->lscale_cap * 4);
+        (void)hipFree(ws->lscale);
+        ws->lscale = nullptr;
+        TRY(dalloc(&ws->lscale, (size_t)cap * 64, &ws->bytes));
+        ws->lscale_cap = cap;
+      } else {
+        TRY(alloc_records(ws, ws->cap * 2));
+      }
       TRY(capture(g, ws, prm));
       continue;
     }
-    while (!ws->hctl->bdone) {
-      HIP_TRY(hipGraphLaunch(ws->gbwd, st));
-      HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    if (!ws->hctl->bdone) {
+      // pick the grid of each backward replay from the sizes of its levels
+      int last = ws->hctl->last;
+      std::vector<int64_t> lo(last + 2);
+      HIP_TRY(hipMemcpyAsync(lo.data(), ws->rec_lo, sizeof(int64_t) * (last + 2), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
+      while (!ws->hctl->bdone) {
+        int top = ws->hctl->bstarted ? ws->hctl->blevel - 1 : last;
+        int bot = std::max(0, top - prm.steps + 1);
+        int64_t recs = lo[std::min(top + 1, last + 1)] - lo[bot];
+        bool sparse = recs < prm.sparse_verts * (int64_t)(top - bot + 1);
+        HIP_TRY(hipGraphLaunch(sparse ? ws->gbwd_s : ws->gbwd, st));
+        HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+      }
     }
     break;
   }
@@ -1855,7 +2001,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
 }
 
 static int sweep_cleanup(const DevGraph& g, Workspace* ws, const Params& prm) {
-  GA ga = make_ga(g);
+  GA ga = make_ga(g, prm);
   if (ws->mode == BCMI_MODE_PLAIN) {
     PA pa = make_pa(ws, prm);
     hipLaunchKernelGGL(k_plain_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, ws->st, pa);
@@ -1963,7 +2109,23 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
       std::vector<size_t> idx(cs.size());
       for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
       const std::vector<uint32_t>& sg = g.h_sig;
-      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return sg[cs[a]] < sg[cs[b]]; });
+      const std::vector<int32_t>& tn = g.h_topnb;
+      if (prm.batching == 3)
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+          int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+          uint32_t ka = ta < INT32_MAX ? sg[ta] : ~0u, kb = tb < INT32_MAX ? sg[tb] : ~0u;
+          if (ka != kb) return ka < kb;
+          if (ta != tb) return ta < tb;
+          return sg[cs[a]] < sg[cs[b]];
+        });
+      else if (prm.batching == 2)
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+          int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+          if (ta != tb) return ta < tb;
+          return sg[cs[a]] < sg[cs[b]];
+        });
+      else if (prm.batching == 1)
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return sg[cs[a]] < sg[cs[b]]; });
       std::vector<int32_t> cs2(cs.size());
       std::vector<double> cm2(cm.size());
       for (size_t i = 0; i < idx.size(); i++) { cs2[i] = cs[idx[i]]; cm2[i] = cm[idx[i]]; }
@@ -1979,7 +2141,15 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
       HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, 64 * sizeof(int32_t), hipMemcpyHostToDevice, st));
       HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, 64 * sizeof(double), hipMemcpyHostToDevice, st));
       int lv = 0;
-      TRY(sweep(g, ws, prm, k, false, &lv));
+      // auto mode: once a batch of this graph has overflowed, later ones start scaled
+      int scaled = prm.sigma == 1 || (prm.sigma == 2 && g.sigma_overflowed.load());
+      TRY(sweep(g, ws, prm, k, false, &lv, scaled));
+      if (ws->hctl->sigma_ovf && prm.sigma == 2) {
+        // exact counts wrapped: the backward pass was skipped, redo the batch scaled
+        g.sigma_overflowed.store(true);
+        TRY(sweep(g, ws, prm, k, false, &lv, 1));
+      }
+      if (scaled || ws->hctl->scaled) R.scaled_batches++;
       HIP_TRY(hipMemcpyAsync(ws->hstats, ws->stats, 3 * 64 * sizeof(u64), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
       for (int q = 0; q < k; q++) {
@@ -1990,6 +2160,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
       }
       if (ws->hctl->sigma_ovf) ovf = 1;
       R.uni_n.push_back(ws->hctl->uni_verts);
+      R.pairs += ws->hctl->tot_verts;
       R.uni_m.push_back(ws->hctl->uni_arcs);
       R.n_td += ws->hctl->n_td; R.n_bu += ws->hctl->n_bu;
       R.sweeps += k;

@@ -72,6 +72,9 @@ typedef struct bcmi_run_stats {
     int64_t sum_n;         /* sum of reached vertices */
     int64_t sum_m;         /* sum of degrees of reached vertices */
     int64_t sum_a;         /* sum of shortest-path DAG arcs */
+    int64_t sum_pairs;     /* batched: (vertex, depth) pairs over batches, i.e. work-list vertices */
+    int64_t sum_union;     /* batched: distinct vertices reached per batch, summed over batches */
+    int64_t scaled_batches; /* batches whose path counts exceeded 2^64-1 and were redone in scaled FP64 */
 } bcmi_run_stats;
 
 /* ---- error reporting ---------------------------------------------------- */
@@ -111,7 +114,12 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * work item), "alpha_plain"/"alpha_batched" (x1000: top-down -> bottom-up
  * switch when frontier arcs > alpha * unvisited arcs), "steps" (levels per
  * captured graph), "width" (sources per batch, 1..64), "ktime" (kernel timed
- * with HIP events, see bcmi_get_kernel_time). */
+ * on the device, see bcmi_get_kernel_time), "sigma" (path counts: 0 exact
+ * uint64, BCMI_ERR_OVERFLOW past 2^64-1; 1 FP64 scaled per level; 2 exact,
+ * with batches that overflow recomputed scaled - the default; the plain
+ * sweep is exact only), "batching" (source order: 0 as given, 1 hub-depth
+ * signature, 2 highest-degree neighbour then signature), "reorder"
+ * (relabel by degree at upload), "kflags" (kernel variants), "trace". */
 int bcmi_set_param(bcmi_ctx *ctx, const char *key, int64_t value);
 int bcmi_get_param(bcmi_ctx *ctx, const char *key, int64_t *value);
 

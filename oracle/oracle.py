@@ -175,3 +175,52 @@ def bc_pipeline(off, adj, sources, heuristic=0, expand=False, nthreads=1):
     if rc != 0:
         raise OracleError(f"orc_bc_pipeline rc={rc}")
     return bc
+
+
+def brandes_highprec(off, adj, sources, digits=50):
+    """Brandes with exact big-integer path counts and a `digits`-digit
+    decimal backward pass (pure Python; small graphs only).  Used where path
+    counts exceed 2^64 and the uint64 oracle stops (long bridged-block
+    chains).  Same rules as sssp/orc_brandes: delta[c] += (delta[r]+1)/
+    sigma[r]*sigma[c] over arcs to the next level (cuda_kernels.cu:590-596).
+    Returns ordered-pair BC as float64."""
+    import decimal
+    ctx = decimal.Context(prec=digits)
+    n = len(off) - 1
+    off = [int(x) for x in off]
+    adj = [int(x) for x in adj]
+    bc = [decimal.Decimal(0)] * n
+    for s in sources:
+        s = int(s)
+        if off[s + 1] == off[s]:
+            continue
+        depth = {s: 0}
+        sigma = {s: 1}
+        levels = [[s]]
+        while True:
+            nxt = []
+            d1 = len(levels)
+            for c in levels[-1]:
+                for k in range(off[c], off[c + 1]):
+                    r = adj[k]
+                    if r not in depth:
+                        depth[r] = d1
+                        sigma[r] = 0
+                        nxt.append(r)
+                    if depth[r] == d1:
+                        sigma[r] += sigma[c]
+            if not nxt:
+                break
+            levels.append(nxt)
+        delta = {}
+        for d in range(len(levels) - 1, 0, -1):
+            for c in levels[d]:
+                acc = decimal.Decimal(0)
+                sc = decimal.Decimal(sigma[c])
+                for k in range(off[c], off[c + 1]):
+                    r = adj[k]
+                    if depth.get(r, -1) == d + 1:
+                        acc = ctx.add(acc, ctx.multiply(ctx.divide(ctx.add(delta[r], 1), decimal.Decimal(sigma[r])), sc))
+                delta[c] = acc
+                bc[c] = ctx.add(bc[c], acc)
+    return np.array([float(x) for x in bc], np.float64)

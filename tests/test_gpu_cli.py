@@ -1,0 +1,74 @@
+"""The drop-in program end to end: the CK run of the reference
+(-p 1x1 -f facebook_combined.txt -n 4039 -N 1000) writes <o>_1X1_0.log with
+the reference's header and %.2f rows of BC/2, here checked against the
+networkx golden vector for sources 0..999."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ck-bc_amd", "ck-bc-hip")
+
+
+def write_facebook(tmp_path):
+    f = np.load(os.path.join(ROOT, "tests", "golden", "facebook.npz"))
+    fn = tmp_path / "facebook_combined.txt"
+    with open(fn, "w") as fp:
+        for a, b in f["edges"].astype(np.int64):
+            fp.write(f"{a} {b}\n")
+    return fn, f
+
+
+def read_result(path):
+    lines = open(path).read().splitlines()
+    assert lines[0] == "BC RESULTS" and lines[1] == "ROWPP 4039" and lines[2] == "NodeId\tBC_VAL"
+    rows = [l.split("\t") for l in lines[3:]]
+    assert [int(r[0]) for r in rows] == list(range(4039))
+    return np.array([float(r[1]) for r in rows])
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_ck_run_facebook(tmp_path, mode):
+    fn, f = write_facebook(tmp_path)
+    out = tmp_path / "res"
+    r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "4039", "-N", "1000", "-o", str(out), "-B", mode],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "BC rounds: 1000" in r.stdout and "------------- RESULTS ---------------" in r.stdout
+    got = read_result(str(out) + "_1X1_0.log")
+    want = np.round(f["bc_src1000"], 2)
+    assert np.max(np.abs(got - want)) <= 0.0100001
+
+
+def test_ck_run_heuristic_full_precision(tmp_path):
+    fn, f = write_facebook(tmp_path)
+    out = tmp_path / "res"
+    r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "4039", "-N", "4039", "-H", "1", "-P",
+                        "-o", str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = read_result(str(out) + "_1X1_0.log")
+    want = f["bc_all"]
+    nz = want != 0
+    assert np.all(got[~nz] == 0)
+    assert np.max(np.abs(got[nz] - want[nz]) / want[nz]) < 1e-9
+
+
+def test_ck_run_errors(tmp_path):
+    fn, _ = write_facebook(tmp_path)
+    r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "100"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "invalid edge" in r.stderr
+    r = subprocess.run([EXE, "-p", "2x2", "-f", str(fn), "-n", "4039"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+
+
+def test_ck_runner(tmp_path):
+    fn, _ = write_facebook(tmp_path)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "ck-bc_amd", "ck_run.py"),
+                        os.path.join(ROOT, "ck-bc_amd", "program", "bc-hip"), "--dataset", str(fn)],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "BC rounds: 1000" in r.stdout

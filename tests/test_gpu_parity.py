@@ -137,9 +137,12 @@ def test_sigma_overflow_reported(engine, mode):
     g = csr(e, n)
     engine.upload(g)
     engine.reset()
+    engine.set_param("sigma", 0)
     with pytest.raises(bcmi.BcmiError) as ex:
         engine.run(np.array([0], np.int32), mode=mode)
+    engine.set_param("sigma", 2)
     assert ex.value.code == bcmi.ERR_OVERFLOW
+    assert np.all(engine.bc() == 0)
     # the engine stays usable
     engine.reset()
     engine.run(np.array([1], np.int32), mode=mode)

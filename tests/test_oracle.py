@@ -264,3 +264,21 @@ def test_planted_mistakes_are_caught():
     _, sg, _, _ = O.sssp(off, adj, 0)
     assert any(int(sg[i * 5 + j]) != 1 for i in range(4) for j in range(5))
     assert int(sg[3 * 5 + 4]) == comb(7, 3)
+
+
+def test_highprec_oracle_matches_uint64_oracle():
+    """The decimal/big-integer Brandes agrees with the uint64 oracle where
+    both apply, and with the path closed form."""
+    off, adj = csr(*G.path(9))
+    assert np.allclose(O.brandes_highprec(off, adj, range(9)), 2.0 * np.arange(9) * (8 - np.arange(9)))
+    e, n = G.random_graph(60, 150, 3)
+    off, adj = csr(e, n)
+    src = [0, 3, 17, 40]
+    a = O.brandes_highprec(off, adj, src)
+    b = O.brandes(off, adj, src)
+    assert np.allclose(a, b, rtol=1e-13, atol=1e-12)
+    off, adj = csr(*G.diamonds(70))        # 2^70 paths: beyond uint64
+    bc = O.brandes_highprec(off, adj, [0])
+    # in a diamond chain every middle vertex of a diamond carries half of the
+    # paths to the vertices after it
+    assert bc[1] == bc[2] and bc[1] > 0
