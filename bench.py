@@ -156,14 +156,22 @@ def main():
     g = bcmi.Graph.rmat(args.scale, args.edgefactor, seed=args.seed)
     log(f"graph scale {args.scale}: n={g.n} arcs={g.m} built in {time.time() - t0:.1f}s")
     width = 64
-    need = (args.warmup + args.steps) * width
-    src = g.sample_sources(max(args.sources, need), seed=args.seed + 1)
+    src = g.sample_sources(args.sources, seed=args.seed + 1)
 
     e = bcmi.Engine(0)
+    import autotune
+    tuned = autotune.apply(e, autotune.load(), autotune.graph_key("rmat", scale=args.scale,
+                                                                  edgefactor=args.edgefactor))
+    if tuned:
+        log(f"tuned parameters: {tuned}")
+        width = int(tuned.get("width", 64))
     t0 = time.time()
     e.upload(g)
     log(f"uploaded in {time.time() - t0:.1f}s")
 
+    need = (args.warmup + args.steps) * width
+    if need > len(src):
+        src = g.sample_sources(need, seed=args.seed + 1)
     kt = {"sigma": 1, "back": 2, "td": 3, "bu": 4}[args.kernel]
     # warmup (also captures the level-loop graphs)
     e.set_param("ktime", kt)
@@ -247,8 +255,10 @@ def main():
         "dtype": "u64 sigma / f64 delta",
         "data": "synthetic R-MAT, seeded",
         "config": {"workload": f"batched Brandes BC, R-MAT scale {args.scale} ef {args.edgefactor} "
-                               f"(a=.57 b=.19 c=.19), {len(timed)} sampled sources in batches of 64",
-                   "n": g.n, "arcs": g.m, "batch": 64},
+                               f"(a=.57 b=.19 c=.19), {len(timed)} sampled sources in batches of {width}",
+                   "n": g.n, "arcs": g.m, "batch": width,
+                   "tuned": {k: v for k, v in (tuned or {}).items() if k in
+                             ("chunk", "block", "grid", "alpha_batched", "sgrid", "width")}},
         "plain_gteps": round(gteps_plain, 3),
         "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain > 0 else None,
         "model_bytes_gbs": round(st["bytes"] / t_batched / 1e9, 1),

@@ -124,6 +124,9 @@ __device__ __forceinline__ u64 ld_coherent_u64(const u64* p) {
 }
 // Adjacency lists are streamed once per level: keep them out of L2 so the
 // masks and rows gathered at random stay resident.
+#ifndef UNR
+#define UNR 8      // row gathers in flight per wave in the sigma and backward loops
+#endif
 #define KF_GATE 1  // gate mask probes with the level bitmaps
 #define KF_NT 2    // non-temporal adjacency loads
 // One bit per vertex: "this vertex has a non-zero mask on this level".
@@ -876,20 +879,20 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
         if (!(g.flags & KF_GATE) || lb_test(clb, u)) F = cur[u] & N;
       }
       u64 P = __ballot(F != 0);
-      // Parents four at a time: the four row loads are issued before any is
+      // Parents UNR at a time: the row loads are issued before any is
       // consumed, and the sums still follow adjacency order.
       while (P) {
-        int q[4];
-        bool ok[4];
+        int q[UNR];
+        bool ok[UNR];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < UNR; t++) {
           ok[t] = P != 0;
           q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
           if (ok[t]) P &= P - 1;
         }
-        u64 x[4];
+        u64 x[UNR];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < UNR; t++) {
           int32_t uq = rl32(u, q[t]);
           u64 fq = rl64(F, q[t]);
           bool use = ok[t] && ((fq >> lane) & 1);
@@ -898,10 +901,10 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
         }
         if (scaled) {
 #pragma unroll
-          for (int t = 0; t < 4; t++) accf = accf + __longlong_as_double((long long)x[t]);
+          for (int t = 0; t < UNR; t++) accf = accf + __longlong_as_double((long long)x[t]);
         } else {
 #pragma unroll
-          for (int t = 0; t < 4; t++) {
+          for (int t = 0; t < UNR; t++) {
             u64 s = acc + x[t];
             if (s < acc) ovf = 1;
             acc = s;
@@ -1011,28 +1014,28 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
         if (!(g.flags & KF_GATE) || lb_test(nlb, z)) M = lmn[z] & L;
       }
       u64 P = __ballot(M != 0);
-      // Successors four at a time (loads first, then the sums in adjacency
+      // Successors UNR at a time (loads first, then the sums in adjacency
       // order; FP contraction is off, so each step rounds like the oracle).
       while (P) {
-        int q[4];
-        bool ok[4];
+        int q[UNR];
+        bool ok[UNR];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < UNR; t++) {
           ok[t] = P != 0;
           q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
           if (ok[t]) P &= P - 1;
         }
-        double x[4];
-        bool use[4];
+        double x[UNR];
+        bool use[UNR];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < UNR; t++) {
           int32_t zq = rl32(z, q[t]);
           u64 mq = rl64(M, q[t]);
           use[t] = ok[t] && ((mq >> lane) & 1);
           x[t] = use[t] ? b.coef[(int64_t)zq * 64 + lane] : 0.0;
         }
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < UNR; t++)
           if (use[t]) acc = acc + x[t] * sv;
       }
     }
