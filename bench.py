@@ -41,10 +41,6 @@ def sync():
         pass
 
 
-def per_source_bytes_batched(st_n, st_m, st_a, uni_n, uni_m):
-    return 24.0 * uni_m + 32.0 * uni_n + float(np.sum(24.0 * st_a + 32.0 * st_n))
-
-
 def serve_bench(args, log):
     """Served BC queries: a pool of graphs of uneven size (R-MAT scales 12-20,
     edge factor 8-32, and bridged-block chains of 256-vertex Erdos-Renyi
@@ -241,6 +237,15 @@ def main():
 
     serving = serve_bench(args, log) if args.serve_queries > 0 else None
 
+    # HBM traffic per launch of the roofline kernel, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_rmat{args.scale}.json")
+    if os.path.exists(tpath) and args.edgefactor == 16:
+        t = json.load(open(tpath)).get("per_kernel", {}).get(f"k_ms_{args.kernel}")
+        if t:
+            traffic = round(t["read_bytes_per_dispatch"] + t["write_bytes_per_dispatch"])
+
     out = {
         "metric": "BC GTEPS on R-MAT scale-22 (batched vs plain, 1 GPU) + p50/p99 query latency",
         "value": round(gteps, 3),
@@ -265,7 +270,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": f"k_ms_{args.kernel}",
+                     "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE)",
+                     "kernel": f"k_ms_{args.kernel}",
                      "avg_launch_us": round(k_ms / max(k_launch, 1) * 1e3, 2)},
         "cpu_baseline": cpu,
         "serving": serving,
