@@ -124,6 +124,9 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "sparse_verts") {
     if (v < 0) return set_error(BCMI_ERR_ARG, "sparse_verts must be >= 0");
     p.sparse_verts = v;
+  } else if (k == "streams") {
+    if (v < 1 || v > 16) return set_error(BCMI_ERR_ARG, "streams must be in [1,16]");
+    p.streams = (int)v;
   } else if (k == "sigma") {
     if (v < 0 || v > 2) return set_error(BCMI_ERR_ARG, "sigma must be 0 (exact), 1 (scaled) or 2 (auto)");
     p.sigma = (int)v;
@@ -167,6 +170,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "sigma") *v = p.sigma;
   else if (k == "sgrid") *v = p.sgrid;
   else if (k == "sparse_verts") *v = p.sparse_verts;
+  else if (k == "streams") *v = p.streams;
   else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
   return BCMI_OK;
 }

@@ -26,8 +26,9 @@ struct Params {
   int64_t alpha_plain = 1000;   // x1000; bottom-up when frontier arcs > alpha * unvisited arcs
   int64_t alpha_batched = 1000; // x1000
   int steps = 16;           // levels per captured graph
-  int sgrid = 128;          // blocks per kernel in the small-grid (sparse-level) graphs
-  int64_t sparse_verts = 4096;  // levels below this many vertices replay the small-grid graphs
+  int sgrid = 512;          // blocks per kernel in the small-grid (sparse-level) graphs
+  int64_t sparse_verts = 16384; // levels below this many vertices replay the small-grid graphs
+  int streams = 4;          // sweeps of one call run at once on deep graphs (own workspace and stream each)
   int width = 64;           // sources per batch
   int sigma = 2;            // path counts: 0 exact uint64 (error on overflow), 1 scaled FP64,
                             // 2 exact, batches that overflow are recomputed scaled

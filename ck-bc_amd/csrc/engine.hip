@@ -316,13 +316,88 @@ __device__ __forceinline__ void app_push(Appender& a, bool want, int32_t v, int6
   a.fill += tot;
 }
 
+// End of an expansion kernel: one reservation per block (not per wave) for
+// the records still staged in LDS, and one update per block of the frontier
+// counters; same-address atomics from every wave serialise, which costs more
+// than the work on small levels.  Every wave of the block must call it.
 __device__ __forceinline__ void app_finish(Appender& a, Ctl* c, int32_t* rec_v, int32_t* rec_j, int64_t cap) {
-  app_flush(a, c, rec_v, rec_j, cap);
+  __shared__ int s_fill[MAX_WAVES_PER_BLOCK];
+  __shared__ long long s_ar[MAX_WAVES_PER_BLOCK], s_vr[MAX_WAVES_PER_BLOCK];
+  __shared__ u64 s_base;
+  const int w = threadIdx.x >> 6, lane = lane_id(), nwv = blockDim.x >> 6;
   long long ar = wave_sum_ll(a.arcs), vr = wave_sum_ll(a.verts);
-  if (lane_id() == 0) {
-    if (ar) atomicAdd((u64*)&c->arcs_next, (u64)ar);
-    if (vr) atomicAdd((u64*)&c->verts_next, (u64)vr);
+  if (lane == 0) { s_fill[w] = a.fill; s_ar[w] = ar; s_vr[w] = vr; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    long long A = 0, V = 0;
+    for (int i = 0; i < nwv; i++) {
+      int f = s_fill[i];
+      s_fill[i] = tot;
+      tot += f;
+      A += s_ar[i];
+      V += s_vr[i];
+    }
+    s_base = tot ? atomicAdd((u64*)&c->rec_count, (u64)tot) : 0;
+    if (A) atomicAdd((u64*)&c->arcs_next, (u64)A);
+    if (V) atomicAdd((u64*)&c->verts_next, (u64)V);
   }
+  __syncthreads();
+  const u64 base = s_base + (u64)s_fill[w];
+  for (int i = lane; i < a.fill; i += 64) {
+    int64_t r = (int64_t)base + i;
+    if (r < cap) {
+      rec_v[r] = a.sv[i];
+      rec_j[r] = a.sj[i];
+    } else {
+      c->overflow = 1;
+    }
+  }
+  a.fill = 0;
+}
+
+// Adds a per-lane value (lane i of every wave -> dst[i]) with one atomic per
+// lane per block.  Every wave of the block must call it.
+__device__ __forceinline__ void block_add_lanes(u64 v, u64* dst) {
+  __shared__ u64 s_v[MAX_WAVES_PER_BLOCK][64];
+  const int w = threadIdx.x >> 6, lane = lane_id(), nwv = blockDim.x >> 6;
+  s_v[w][lane] = v;
+  __syncthreads();
+  if (w == 0) {
+    u64 t = 0;
+    for (int i = 0; i < nwv; i++) t += s_v[i][lane];
+    if (t) atomicAdd(dst + lane, t);
+  }
+  __syncthreads();
+}
+
+// Per-lane maximum of non-negative doubles given as their bit patterns (which
+// order like the values), one atomic per lane per block.
+__device__ __forceinline__ void block_max_lanes(u64 v, u64* dst) {
+  __shared__ u64 s_m[MAX_WAVES_PER_BLOCK][64];
+  const int w = threadIdx.x >> 6, lane = lane_id(), nwv = blockDim.x >> 6;
+  s_m[w][lane] = v;
+  __syncthreads();
+  if (w == 0) {
+    u64 t = 0;
+    for (int i = 0; i < nwv; i++) t = s_m[i][lane] > t ? s_m[i][lane] : t;
+    if (t) atomicMax(dst + lane, t);
+  }
+  __syncthreads();
+}
+
+// Adds a wave-uniform value once per block.  Every wave must call it.
+__device__ __forceinline__ void block_add_uniform(long long v, u64* dst) {
+  __shared__ long long s_u[MAX_WAVES_PER_BLOCK];
+  const int w = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  if (lane_id() == 0) s_u[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int i = 0; i < nwv; i++) t += s_u[i];
+    if (t) atomicAdd(dst, (u64)t);
+  }
+  __syncthreads();
 }
 
 #define APPENDER(a)                                        \
@@ -637,7 +712,7 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled) {
   int32_t s = i < k ? b.srcbuf[i] : 0;
   if (i < k) {
     atomicOr(&lm0[s], 1ull << i);
-    lb_set(b.lb, s);
+    if (g.flags & KF_GATE) lb_set(b.lb, s);
     atomicOr(&b.seen[s], 1ull << i);
     atomicOr(&b.compmask[g.label[s]], 1ull << i);
     b.sigma[(int64_t)s * 64 + i] = scaled ? (u64)__double_as_longlong(1.0) : 1ull;
@@ -761,7 +836,7 @@ __device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
           won = (old == 0);
           if (won) {
             dg = g.off[v + 1] - g.off[v];
-            lb_set(nlb, v);
+            if (g.flags & KF_GATE) lb_set(nlb, v);
           }
         }
       }
@@ -814,7 +889,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         won = (old == 0);
         if (won) {
           dg = g.off[v + 1] - g.off[v];
-          lb_set(nlb, v);
+          if (g.flags & KF_GATE) lb_set(nlb, v);
         }
       }
     }
@@ -949,17 +1024,17 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
          r += (int64_t)gridDim.x * blockDim.x)
       if (b.rec_j[r] == 0) {
         old[b.rec_v[r]] = 0;
-        lb_clr(olb, b.rec_v[r]);
+        if (g.flags & KF_GATE) lb_clr(olb, b.rec_v[r]);
       }
   }
-  if (cn) atomicAdd(&b.st_n[lane], cn);
-  if (cm) atomicAdd(&b.st_m[lane], cm);
-  if (ca) atomicAdd(&b.st_a[lane], ca);
-  if (done_arcs) atomicAdd((u64*)&c->arcs_seen, (u64)done_arcs);
-  if (uv) atomicAdd((u64*)&c->uni_verts, (u64)uv);
-  if (ua) atomicAdd((u64*)&c->uni_arcs, (u64)ua);
+  block_add_lanes(cn, b.st_n);
+  block_add_lanes(cm, b.st_m);
+  block_add_lanes(ca, b.st_a);
+  block_add_uniform(done_arcs, (u64*)&c->arcs_seen);
+  block_add_uniform(uv, (u64*)&c->uni_verts);
+  block_add_uniform(ua, (u64*)&c->uni_arcs);
   if (ovf) c->sigma_ovf = 1;
-  if (scaled && mx > 0.0) atomicMax(&b.lvmax[lane], (u64)__double_as_longlong(mx));
+  if (scaled) block_max_lanes((u64)__double_as_longlong(mx), b.lvmax);
 }
 
 __global__ void __launch_bounds__(256) k_ms_sigma(GA g, BA b, KT* kt) {
@@ -1054,7 +1129,7 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
     }
     if (j == 0 && lane == 0) {
       lmd[v] = L;
-      lb_set(dlb, v);
+      if (g.flags & KF_GATE) lb_set(dlb, v);
     }
   }
   {
@@ -1066,7 +1141,7 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
            r += (int64_t)gridDim.x * blockDim.x)
         if (b.rec_j[r] == 0) {
           old[b.rec_v[r]] = 0;
-          lb_clr(olb, b.rec_v[r]);
+          if (g.flags & KF_GATE) lb_clr(olb, b.rec_v[r]);
         }
   }
 }
@@ -1121,9 +1196,9 @@ __global__ void k_ms_cleanup(GA g, BA b) {
       b.lm[v] = 0;
       b.lm[b.n + v] = 0;
       b.lm[2 * b.n + v] = 0;
-      lb_clr(b.lb, v);
-      lb_clr(b.lb + b.nw, v);
-      lb_clr(b.lb + 2 * b.nw, v);
+      if (g.flags & KF_GATE) lb_clr(b.lb, v);
+      if (g.flags & KF_GATE) lb_clr(b.lb + b.nw, v);
+      if (g.flags & KF_GATE) lb_clr(b.lb + 2 * b.nw, v);
       b.compmask[g.label[v]] = 0;
     }
   }
@@ -1352,7 +1427,8 @@ void DevGraph::release() {
 // four highest-degree vertices.  Sources with equal keys have similar depth
 // profiles, so a batch of them shares levels: its per-vertex masks are dense
 // and each vertex appears on few levels.
-static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std::vector<uint32_t>& sig) {
+static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std::vector<uint32_t>& sig,
+                          int* max_depth) {
   const int H = 4;
   std::vector<int32_t> hubs;
   {
@@ -1368,6 +1444,7 @@ static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std
     for (auto& x : best) hubs.push_back(x.second);
   }
   std::vector<std::vector<uint8_t>> dep(H, std::vector<uint8_t>(n, 255));
+  std::vector<int> hmax(H, 0);
   std::vector<std::thread> th;
   for (int h = 0; h < (int)hubs.size(); h++)
     th.emplace_back([&, h] {
@@ -1383,8 +1460,11 @@ static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std
         for (int64_t e = off[u]; e < off[u + 1]; e++)
           if (D[adj[e]] == 255) { D[adj[e]] = du + 1; q.push_back(adj[e]); }
       }
+      hmax[h] = D[q.back()];
     });
   for (auto& t : th) t.join();
+  *max_depth = 0;
+  for (int h = 0; h < (int)hubs.size(); h++) *max_depth = std::max(*max_depth, hmax[h]);
   sig.assign(n, 0);
   for (int64_t v = 0; v < n; v++) {
     uint32_t k = 0;
@@ -1534,7 +1614,7 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
     HIP_TRY(hipStreamSynchronize(st));
   }
   g.h_off = hoff;
-  hub_signature(n, off, adj, g.h_sig);
+  hub_signature(n, off, adj, g.h_sig, &g.depth_hint);
   // first neighbour in descending-degree order (the smallest internal id once relabelled)
   g.h_topnb.assign(n, INT32_MAX);
   for (int64_t v = 0; v < n; v++)
@@ -1633,6 +1713,10 @@ struct Workspace {
   Params cap_prm;
   KT* kt = nullptr;             // device timer of the kernel selected by Params::ktime
   KT* hkt = nullptr;            // pinned mirror
+  // Deep graphs (long chains): further workspaces, each with its own stream
+  // and BC array, that run sweeps of the same call concurrently.
+  std::vector<Workspace*> helpers;
+  const DevGraph* helpers_graph = nullptr;
   const DevGraph* cap_graph = nullptr;
   size_t bytes = 0;
 };
@@ -1667,21 +1751,32 @@ static BA make_ba(Workspace* ws, const Params& prm) {
 
 static int64_t record_capacity(const DevGraph& g, int mode) {
   // Every vertex appears at most once per distinct depth; plain sweeps at
-  // most once.  Start with room for a few depths per vertex; the run grows
-  // the buffer when a sweep reports overflow.
+  // most once.  A batch's sources sit at up to 64 distinct distances from a
+  // vertex: few on shallow graphs (their depths coincide), up to all 64 on
+  // deep ones (long chains).  Start with room for depth/2 depths per vertex
+  // (the hubs' BFS depth, at least 4); a sweep that overflows grows it.
   int64_t base = g.n + g.m / g.chunk + 1024;
-  return mode == BCMI_MODE_PLAIN ? base : 4 * base;
+  int64_t mult = std::min<int64_t>(64, std::max<int64_t>(4, g.depth_hint / 2));
+  return mode == BCMI_MODE_PLAIN ? base : mult * base;
+}
+
+// Sweeps of one call that run at once on deep graphs (hub BFS depth >= 64):
+// their levels are small and latency-bound, so several sweeps on their own
+// streams overlap well.  Shallow graphs keep the device busy with one.
+static int concurrent_sweeps(const DevGraph& g, const Params& prm) {
+  return g.depth_hint >= 64 ? std::max(1, std::min(prm.streams, 16)) : 1;
 }
 
 size_t workspace_bytes(const DevGraph& g, int mode, int width, const Params& prm) {
-  (void)width; (void)prm;
+  (void)width;
+  const size_t copies = (size_t)concurrent_sweeps(g, prm);
   int64_t n = g.n;
   int64_t cap = record_capacity(g, mode);
   size_t b = sizeof(double) * n + sizeof(int64_t) * (n + 3) + sizeof(int32_t) * n + 2 * sizeof(int32_t) * cap;
   b += 4 * sizeof(u64) * n;  // per-run source terms
   if (mode == BCMI_MODE_PLAIN) b += (sizeof(int32_t) + sizeof(u64) + 2 * sizeof(double)) * n;
   else b += sizeof(u64) * cap + (5 * sizeof(u64) + 64 * (sizeof(u64) + sizeof(double))) * n;
-  return b;
+  return b * copies;
 }
 
 static void free_graphs(Workspace* ws) {
@@ -1703,6 +1798,8 @@ __global__ void k_kt_reset(KT* kt) {
 void workspace_destroy(Workspace* ws) {
   ALLOC_LOCK;
   if (!ws) return;
+  for (Workspace* h : ws->helpers) workspace_destroy(h);
+  ws->helpers.clear();
   free_graphs(ws);
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   if (ws->own_bc) fr(ws->bc);
@@ -1807,7 +1904,11 @@ Workspace* workspace_create(const DevGraph& g, int mode, int width, const Params
   return ws;
 }
 hipStream_t workspace_stream(Workspace* ws) { return ws->st; }
-size_t workspace_size(Workspace* ws) { return ws->bytes; }
+size_t workspace_size(Workspace* ws) {
+  size_t b = ws->bytes;
+  for (Workspace* h : ws->helpers) b += h->bytes;
+  return b;
+}
 void workspace_share_bc(Workspace* ws, double* bc) {
   ALLOC_LOCK;
   if (ws->own_bc && ws->bc) (void)hipFree(ws->bc);
@@ -1944,7 +2045,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       HIP_TRY(hipStreamSynchronize(st));
     }
     if (ws->hctl->overflow) {
-      // Work list too small: reset every state array and retry with twice the room.
+      // Work list too small: reset every state array and retry with four times the room.
       int64_t n = ws->n;
       int G = grid_for(n);
       if (ws->mode == BCMI_MODE_PLAIN) {
@@ -1967,7 +2068,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
         TRY(dalloc(&ws->lscale, (size_t)cap * 64, &ws->bytes));
         ws->lscale_cap = cap;
       } else {
-        TRY(alloc_records(ws, ws->cap * 2));
+        TRY(alloc_records(ws, ws->cap * 4));
       }
       TRY(capture(g, ws, prm));
       continue;
@@ -2071,6 +2172,99 @@ static int prepare_sources(const DevGraph& g, Workspace* ws, const int32_t* src,
   return BCMI_OK;
 }
 
+// Statistics and outcome of the sweeps one thread drove.
+struct RunPart {
+  RunResult R;
+  int ovf = 0;
+  int rc = BCMI_OK;
+  std::string msg;
+};
+
+// One plain sweep from source s (internal id) with BC multiplier mult.
+static int run_plain_one(const DevGraph& g, Workspace* w, const Params& prm, int32_t s, double mult, RunPart& P) {
+  hipStream_t st = w->st;
+  w->hsrc[0] = s;
+  w->hmult[0] = mult;
+  HIP_TRY(hipMemcpyAsync(w->srcbuf, w->hsrc, sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(w->multbuf, w->hmult, sizeof(double), hipMemcpyHostToDevice, st));
+  int lv = 0;
+  TRY(sweep(g, w, prm, 1, false, &lv));
+  const Ctl& c = *w->hctl;
+  RunResult& R = P.R;
+  R.n_s.push_back(c.tot_verts);
+  R.m_s.push_back(c.tot_arcs);
+  R.a_s.push_back(c.sum_a);
+  R.n_td += c.n_td; R.n_bu += c.n_bu;
+  R.sweeps++; R.launches++;
+  R.max_levels = std::max<int64_t>(R.max_levels, lv);
+  R.levels.push_back(lv);
+  if (c.sigma_ovf) P.ovf = 1;
+  return BCMI_OK;
+}
+
+// One batched sweep over k <= 64 sources (internal ids) with multipliers.
+static int run_batch(const DevGraph& g, Workspace* w, const Params& prm, const int32_t* src, const double* mult,
+                     int k, RunPart& P) {
+  hipStream_t st = w->st;
+  for (int q = 0; q < 64; q++) {
+    w->hsrc[q] = q < k ? src[q] : 0;
+    w->hmult[q] = q < k ? mult[q] : 0.0;
+  }
+  HIP_TRY(hipMemcpyAsync(w->srcbuf, w->hsrc, 64 * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(w->multbuf, w->hmult, 64 * sizeof(double), hipMemcpyHostToDevice, st));
+  int lv = 0;
+  // auto mode: once a batch of this graph has overflowed, later ones start scaled
+  int scaled = prm.sigma == 1 || (prm.sigma == 2 && g.sigma_overflowed.load());
+  TRY(sweep(g, w, prm, k, false, &lv, scaled));
+  if (w->hctl->sigma_ovf && prm.sigma == 2) {
+    // exact counts wrapped: the backward pass was skipped, redo the batch scaled
+    g.sigma_overflowed.store(true);
+    TRY(sweep(g, w, prm, k, false, &lv, 1));
+  }
+  RunResult& R = P.R;
+  if (scaled || w->hctl->scaled) R.scaled_batches++;
+  HIP_TRY(hipMemcpyAsync(w->hstats, w->stats, 3 * 64 * sizeof(u64), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int q = 0; q < k; q++) {
+    R.n_s.push_back((int64_t)w->hstats[q]);
+    R.m_s.push_back((int64_t)w->hstats[64 + q]);
+    R.a_s.push_back((int64_t)w->hstats[128 + q]);
+    R.levels.push_back(lv);
+  }
+  if (w->hctl->sigma_ovf) P.ovf = 1;
+  R.uni_n.push_back(w->hctl->uni_verts);
+  R.pairs += w->hctl->tot_verts;
+  R.uni_m.push_back(w->hctl->uni_arcs);
+  R.n_td += w->hctl->n_td; R.n_bu += w->hctl->n_bu;
+  R.sweeps += k;
+  R.launches++;
+  R.max_levels = std::max<int64_t>(R.max_levels, lv);
+  return BCMI_OK;
+}
+
+__global__ void k_add_take_f64(double* dst, double* src, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    dst[i] += src[i];
+    src[i] = 0.0;
+  }
+}
+
+// Makes sure ws has `count` helper workspaces for graph g.
+static int ensure_helpers(const DevGraph& g, Workspace* ws, const Params& prm, int count) {
+  if (ws->helpers_graph != &g) {
+    for (Workspace* h : ws->helpers) workspace_destroy(h);
+    ws->helpers.clear();
+    ws->helpers_graph = &g;
+  }
+  while ((int)ws->helpers.size() < count) {
+    int rc = BCMI_OK;
+    Workspace* h = workspace_create(g, ws->mode, ws->width, prm, &rc);
+    if (!h) return rc;
+    ws->helpers.push_back(h);
+  }
+  return BCMI_OK;
+}
+
 int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* sources, int64_t nsrc, int mode,
            int src_flags, RunResult* out) {
   if (mode != ws->mode) return set_error(BCMI_ERR_STATE, "workspace was created for another mode");
@@ -2081,95 +2275,105 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   std::vector<double> cm;
   bool fin = false;
   TRY(prepare_sources(g, ws, sources, nsrc, src_flags, cs, cm, fin));
-  RunResult R;
   hipStream_t st = ws->st;
   hipLaunchKernelGGL(k_kt_reset, 1, 1, 0, st, ws->kt);
   HIP_TRY(hipEventRecord(ws->ev0, st));
-  int ovf = 0;
-  if (mode == BCMI_MODE_PLAIN) {
-    for (size_t i = 0; i < cs.size(); i++) {
-      ws->hsrc[0] = cs[i];
-      ws->hmult[0] = cm[i];
-      HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, sizeof(int32_t), hipMemcpyHostToDevice, st));
-      HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, sizeof(double), hipMemcpyHostToDevice, st));
-      int lv = 0;
-      TRY(sweep(g, ws, prm, 1, false, &lv));
-      const Ctl& c = *ws->hctl;
-      R.n_s.push_back(c.tot_verts);
-      R.m_s.push_back(c.tot_arcs);
-      R.a_s.push_back(c.sum_a);
-      R.n_td += c.n_td; R.n_bu += c.n_bu;
-      R.sweeps++; R.launches++;
-      R.max_levels = std::max<int64_t>(R.max_levels, lv);
-      R.levels.push_back(lv);
-      if (c.sigma_ovf) ovf = 1;
-    }
-  } else {
-    int W = std::max(1, std::min(64, prm.width));
+  const bool plain = mode == BCMI_MODE_PLAIN;
+  const int W = std::max(1, std::min(64, prm.width));
+  if (!plain) {
     // Batch sources with similar depth profiles together (BC is a sum over
     // sources, so the order only changes the rounding of the final sums).
-    {
-      std::vector<size_t> idx(cs.size());
-      for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
-      const std::vector<uint32_t>& sg = g.h_sig;
-      const std::vector<int32_t>& tn = g.h_topnb;
-      if (prm.batching == 3)
-        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-          int32_t ta = tn[cs[a]], tb = tn[cs[b]];
-          uint32_t ka = ta < INT32_MAX ? sg[ta] : ~0u, kb = tb < INT32_MAX ? sg[tb] : ~0u;
-          if (ka != kb) return ka < kb;
-          if (ta != tb) return ta < tb;
-          return sg[cs[a]] < sg[cs[b]];
-        });
-      else if (prm.batching == 2)
-        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-          int32_t ta = tn[cs[a]], tb = tn[cs[b]];
-          if (ta != tb) return ta < tb;
-          return sg[cs[a]] < sg[cs[b]];
-        });
-      else if (prm.batching == 1)
-        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return sg[cs[a]] < sg[cs[b]]; });
-      std::vector<int32_t> cs2(cs.size());
-      std::vector<double> cm2(cm.size());
-      for (size_t i = 0; i < idx.size(); i++) { cs2[i] = cs[idx[i]]; cm2[i] = cm[idx[i]]; }
-      cs.swap(cs2);
-      cm.swap(cm2);
+    std::vector<size_t> idx(cs.size());
+    for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+    const std::vector<uint32_t>& sg = g.h_sig;
+    const std::vector<int32_t>& tn = g.h_topnb;
+    if (prm.batching == 3)
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+        uint32_t ka = ta < INT32_MAX ? sg[ta] : ~0u, kb = tb < INT32_MAX ? sg[tb] : ~0u;
+        if (ka != kb) return ka < kb;
+        if (ta != tb) return ta < tb;
+        return sg[cs[a]] < sg[cs[b]];
+      });
+    else if (prm.batching == 2)
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+        if (ta != tb) return ta < tb;
+        return sg[cs[a]] < sg[cs[b]];
+      });
+    else if (prm.batching == 1)
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return sg[cs[a]] < sg[cs[b]]; });
+    std::vector<int32_t> cs2(cs.size());
+    std::vector<double> cm2(cm.size());
+    for (size_t i = 0; i < idx.size(); i++) { cs2[i] = cs[idx[i]]; cm2[i] = cm[idx[i]]; }
+    cs.swap(cs2);
+    cm.swap(cm2);
+  }
+  const size_t nitems = plain ? cs.size() : (cs.size() + W - 1) / W;
+  const int K = (int)std::min<size_t>((size_t)concurrent_sweeps(g, prm), std::max<size_t>(1, nitems));
+  if (K > 1) TRY(ensure_helpers(g, ws, prm, K - 1));
+  std::vector<RunPart> parts(K);
+  std::atomic<size_t> next{0};
+  std::atomic<bool> stop{false};
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  auto work = [&](int t) {
+    RunPart& P = parts[t];
+    if (t > 0 && hipSetDevice(dev) != hipSuccess) {
+      P.rc = BCMI_ERR_HIP;
+      P.msg = "hipSetDevice failed in a sweep thread";
+      stop = true;
+      return;
     }
-    for (size_t i = 0; i < cs.size(); i += W) {
-      int k = (int)std::min<size_t>(W, cs.size() - i);
-      for (int q = 0; q < 64; q++) {
-        ws->hsrc[q] = q < k ? cs[i + q] : 0;
-        ws->hmult[q] = q < k ? cm[i + q] : 0.0;
+    Workspace* w = t == 0 ? ws : ws->helpers[t - 1];
+    while (!stop) {
+      size_t it = next.fetch_add(1);
+      if (it >= nitems) break;
+      int rc;
+      if (plain) {
+        rc = run_plain_one(g, w, prm, cs[it], cm[it], P);
+      } else {
+        size_t i0 = it * W;
+        int k = (int)std::min<size_t>(W, cs.size() - i0);
+        rc = run_batch(g, w, prm, cs.data() + i0, cm.data() + i0, k, P);
       }
-      HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, 64 * sizeof(int32_t), hipMemcpyHostToDevice, st));
-      HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, 64 * sizeof(double), hipMemcpyHostToDevice, st));
-      int lv = 0;
-      // auto mode: once a batch of this graph has overflowed, later ones start scaled
-      int scaled = prm.sigma == 1 || (prm.sigma == 2 && g.sigma_overflowed.load());
-      TRY(sweep(g, ws, prm, k, false, &lv, scaled));
-      if (ws->hctl->sigma_ovf && prm.sigma == 2) {
-        // exact counts wrapped: the backward pass was skipped, redo the batch scaled
-        g.sigma_overflowed.store(true);
-        TRY(sweep(g, ws, prm, k, false, &lv, 1));
+      if (rc != BCMI_OK) {
+        P.rc = rc;
+        P.msg = bcmi_last_error();
+        stop = true;
+        break;
       }
-      if (scaled || ws->hctl->scaled) R.scaled_batches++;
-      HIP_TRY(hipMemcpyAsync(ws->hstats, ws->stats, 3 * 64 * sizeof(u64), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-      for (int q = 0; q < k; q++) {
-        R.n_s.push_back((int64_t)ws->hstats[q]);
-        R.m_s.push_back((int64_t)ws->hstats[64 + q]);
-        R.a_s.push_back((int64_t)ws->hstats[128 + q]);
-        R.levels.push_back(lv);
-      }
-      if (ws->hctl->sigma_ovf) ovf = 1;
-      R.uni_n.push_back(ws->hctl->uni_verts);
-      R.pairs += ws->hctl->tot_verts;
-      R.uni_m.push_back(ws->hctl->uni_arcs);
-      R.n_td += ws->hctl->n_td; R.n_bu += ws->hctl->n_bu;
-      R.sweeps += k;
-      R.launches++;
-      R.max_levels = std::max<int64_t>(R.max_levels, lv);
     }
+  };
+  {
+    std::vector<std::thread> th;
+    for (int t = 1; t < K; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+  }
+  RunResult R;
+  int ovf = 0;
+  for (RunPart& P : parts) {
+    if (P.rc != BCMI_OK) return set_error(P.rc, P.msg);
+    RunResult& Q = P.R;
+    R.n_s.insert(R.n_s.end(), Q.n_s.begin(), Q.n_s.end());
+    R.m_s.insert(R.m_s.end(), Q.m_s.begin(), Q.m_s.end());
+    R.a_s.insert(R.a_s.end(), Q.a_s.begin(), Q.a_s.end());
+    R.levels.insert(R.levels.end(), Q.levels.begin(), Q.levels.end());
+    R.uni_n.insert(R.uni_n.end(), Q.uni_n.begin(), Q.uni_n.end());
+    R.uni_m.insert(R.uni_m.end(), Q.uni_m.begin(), Q.uni_m.end());
+    R.pairs += Q.pairs;
+    R.n_td += Q.n_td; R.n_bu += Q.n_bu;
+    R.sweeps += Q.sweeps; R.launches += Q.launches;
+    R.scaled_batches += Q.scaled_batches;
+    R.max_levels = std::max(R.max_levels, Q.max_levels);
+    ovf |= P.ovf;
+  }
+  // fold the helpers' BC into this workspace's
+  for (int t = 1; t < K; t++) {
+    Workspace* h = ws->helpers[t - 1];
+    HIP_TRY(hipStreamSynchronize(h->st));
+    hipLaunchKernelGGL(k_add_take_f64, grid_for(g.n), 256, 0, st, ws->bc, h->bc, g.n);
   }
   if (fin) {
     hipLaunchKernelGGL(k_finalize, grid_for(g.n), 256, 0, st, g.label, g.round, (const u64*)g.w, ws->cnt0, ws->cntS,
@@ -2185,8 +2389,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   HIP_TRY(hipMemcpyAsync(ws->hkt, ws->kt, sizeof(KT), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   {
-    int dev = 0, khz = 0;
-    HIP_TRY(hipGetDevice(&dev));
+    int khz = 0;
     HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     R.ktime_ms = khz > 0 ? (double)ws->hkt->total / (double)khz : 0.0;
     R.klaunches = (int64_t)ws->hkt->launches;

@@ -119,7 +119,11 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * with batches that overflow recomputed scaled - the default; the plain
  * sweep is exact only), "batching" (source order: 0 as given, 1 hub-depth
  * signature, 2 highest-degree neighbour then signature), "reorder"
- * (relabel by degree at upload), "kflags" (kernel variants), "trace". */
+ * (relabel by degree at upload), "kflags" (kernel variants), "sgrid"
+ * (blocks per kernel on small levels), "sparse_verts" (levels with fewer
+ * vertices use the small grid), "streams" (sweeps of one call run at once,
+ * each on its own stream and workspace, on graphs whose hub BFS depth is
+ * 64 or more), "trace". */
 int bcmi_set_param(bcmi_ctx *ctx, const char *key, int64_t value);
 int bcmi_get_param(bcmi_ctx *ctx, const char *key, int64_t *value);
 
