@@ -10,8 +10,16 @@ Printed JSON line (one): GTEPS of the batched path (traversed edges = 2 *
 sum over sources of the degree sum of the vertices each source reaches),
 the plain one-source-per-sweep GTEPS on the first `--plain-sources` of the
 same sources, the roofline of the dominant kernel (algorithmic bytes from the
-frozen byte model / kernel time measured with HIP events inside the captured
-level loop) and the CPU oracle timed on a bounded sample.
+frozen byte model / kernel duration from a device-side timer inside the
+captured level loop), the HBM traffic per launch of that kernel from the
+committed rocprofv3 counter runs, the CPU oracle timed on a bounded sample
+and a served-query run (p50/p99 latency).
+
+Other workloads (not the default line):
+  --scale 18 --pendant 0.3 --sources 8192     R-MAT 18 + 30% pendant trees, peeled
+  --scale 25 --sources 4096 --steps 62        R-MAT scale 25
+  --graph chain --chain-blocks 32768 --sources 1024 --steps 12 --warmup 4 --kernel none
+                                              bridged-block chain, 8.4M vertices
 """
 import argparse
 import json
@@ -119,7 +127,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=20250101)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", default="back", choices=["back", "sigma", "td", "bu"])
+    ap.add_argument("--kernel", default="back", choices=["back", "sigma", "td", "bu", "none"],
+                    help="kernel timed on the device for the roofline (none: no device timer)")
+    ap.add_argument("--graph", default="rmat", choices=["rmat", "chain"],
+                    help="chain: bridged-block chain of --chain-blocks Erdos-Renyi blocks of 256 vertices, degree 8")
+    ap.add_argument("--chain-blocks", type=int, default=32768)
+    ap.add_argument("--pendant", type=float, default=0.0,
+                    help="attach pendant trees adding this fraction of vertices and peel them (-H 1, exact sources)")
     ap.add_argument("--serve-queries", type=int, default=48, help="0 skips the serving run")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (queries/s)")
     ap.add_argument("--serve-graphs", type=int, default=32)
@@ -149,26 +163,37 @@ def main():
     import bcmi
 
     t0 = time.time()
-    g = bcmi.Graph.rmat(args.scale, args.edgefactor, seed=args.seed)
-    log(f"graph scale {args.scale}: n={g.n} arcs={g.m} built in {time.time() - t0:.1f}s")
+    if args.graph == "chain":
+        g = bcmi.Graph.block_chain(args.chain_blocks, 256, 8.0, seed=args.seed)
+        gdesc = f"bridged-block chain of {args.chain_blocks} blocks x 256 vertices (degree 8)"
+    else:
+        g = bcmi.Graph.rmat(args.scale, args.edgefactor, seed=args.seed)
+        gdesc = f"R-MAT scale {args.scale} ef {args.edgefactor} (a=.57 b=.19 c=.19)"
+    if args.pendant > 0:
+        g = g.with_pendant_trees(args.pendant, seed=args.seed + 2)
+        gdesc += f" + pendant trees ({args.pendant:.0%} more vertices, peeled)"
+    heur = bcmi.H_PEEL if args.pendant > 0 else bcmi.H_NONE
+    log(f"graph {gdesc}: n={g.n} arcs={g.m} built in {time.time() - t0:.1f}s")
     width = 64
     src = g.sample_sources(args.sources, seed=args.seed + 1)
 
     e = bcmi.Engine(0)
     import autotune
-    tuned = autotune.apply(e, autotune.load(), autotune.graph_key("rmat", scale=args.scale,
-                                                                  edgefactor=args.edgefactor))
+    tuned = None
+    if args.graph == "rmat":
+        tuned = autotune.apply(e, autotune.load(), autotune.graph_key("rmat", scale=args.scale,
+                                                                      edgefactor=args.edgefactor))
     if tuned:
         log(f"tuned parameters: {tuned}")
         width = int(tuned.get("width", 64))
     t0 = time.time()
-    e.upload(g)
+    e.upload(g, heuristic=heur)
     log(f"uploaded in {time.time() - t0:.1f}s")
 
     need = (args.warmup + args.steps) * width
     if need > len(src):
         src = g.sample_sources(need, seed=args.seed + 1)
-    kt = {"sigma": 1, "back": 2, "td": 3, "bu": 4}[args.kernel]
+    kt = {"sigma": 1, "back": 2, "td": 3, "bu": 4, "none": 0}[args.kernel]
     # warmup (also captures the level-loop graphs)
     e.set_param("ktime", kt)
     if args.warmup:
@@ -202,36 +227,44 @@ def main():
         kbytes = union_part / 2.0 + float(np.sum(8.0 * a_s + 16.0 * n_s))
     else:
         kbytes = union_part / 2.0
-    achieved = kbytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else None
+    achieved = kbytes / (k_ms / 1e3) / 1e9 if k_ms > 0 and kt else None
     log(f"kernel {args.kernel}: {k_launch} launches, {k_ms:.1f} ms total, "
         f"avg {k_ms / max(k_launch, 1) * 1e3:.1f} us, achieved {achieved} GB/s")
 
     # plain path on the same (first) sources
     e.set_param("ktime", 0)
     plain_src = timed[:args.plain_sources]
-    e.run(plain_src[:4], mode=bcmi.MODE_PLAIN)  # warm (capture)
-    e.reset()
-    sync()
-    t0 = time.perf_counter()
-    stp = e.run(plain_src, mode=bcmi.MODE_PLAIN)
-    sync()
-    t_plain = time.perf_counter() - t0
-    gteps_plain = stp["te"] / t_plain / 1e9
-    log(f"plain: {len(plain_src)} sources (the first of the timed list) in {t_plain:.3f}s "
-        f"-> {gteps_plain:.2f} GTEPS")
+    gteps_plain = None
+    try:
+        e.run(plain_src[:4], mode=bcmi.MODE_PLAIN)  # warm (capture)
+        e.reset()
+        sync()
+        t0 = time.perf_counter()
+        stp = e.run(plain_src, mode=bcmi.MODE_PLAIN)
+        sync()
+        t_plain = time.perf_counter() - t0
+        gteps_plain = stp["te"] / t_plain / 1e9
+        log(f"plain: {len(plain_src)} sources (the first of the timed list) in {t_plain:.3f}s "
+            f"-> {gteps_plain:.2f} GTEPS")
+    except bcmi.BcmiError as ex:
+        if ex.code != bcmi.ERR_OVERFLOW:
+            raise
+        log("plain: path counts exceed 2^64 on this graph (the plain sweep is exact-only)")
 
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
         nth = args.cpu_threads or min(16, os.cpu_count() or 1)
         cs = timed[:args.cpu_sources]
-        O.brandes(g.off, g.adj, cs[:1], nthreads=1)  # page in
+        # plain Brandes on the uploaded graph (no peeling); on chains the uint64
+        # path counts wrap, which changes no work: the run is timed only
+        O.brandes(g.off, g.adj, cs[:1], nthreads=1, allow_overflow=True)  # page in
         t0 = time.perf_counter()
-        _, ost = O.brandes(g.off, g.adj, cs, nthreads=nth, stats=True)
+        _, ost = O.brandes(g.off, g.adj, cs, nthreads=nth, stats=True, allow_overflow=True)
         t_cpu = time.perf_counter() - t0
         te_cpu = 2.0 * float(np.sum(ost["m_s"]))
         cpu = {"value": round(te_cpu / t_cpu / 1e9, 4), "unit": "GTEPS", "cores": nth, "kind": "port",
-               "sample": f"{len(cs)} of the timed sources on the same scale-{args.scale} graph, "
+               "sample": f"{len(cs)} of the timed sources on the same graph ({gdesc}), "
                          f"{t_cpu:.1f}s, OpenMP over sources"}
         log(f"cpu oracle: {cpu}")
 
@@ -241,7 +274,7 @@ def main():
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_rmat{args.scale}.json")
-    if os.path.exists(tpath) and args.edgefactor == 16:
+    if os.path.exists(tpath) and args.edgefactor == 16 and args.graph == "rmat" and args.pendant == 0:
         t = json.load(open(tpath)).get("per_kernel", {}).get(f"k_ms_{args.kernel}")
         if t:
             traffic = round(t["read_bytes_per_dispatch"] + t["write_bytes_per_dispatch"])
@@ -258,20 +291,19 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64 sigma / f64 delta",
-        "data": "synthetic R-MAT, seeded",
-        "config": {"workload": f"batched Brandes BC, R-MAT scale {args.scale} ef {args.edgefactor} "
-                               f"(a=.57 b=.19 c=.19), {len(timed)} sampled sources in batches of {width}",
+        "data": "synthetic, seeded",
+        "config": {"workload": f"batched Brandes BC, {gdesc}, {len(timed)} sampled sources in batches of {width}",
                    "n": g.n, "arcs": g.m, "batch": width,
                    "tuned": {k: v for k, v in (tuned or {}).items() if k in
                              ("chunk", "block", "grid", "alpha_batched", "sgrid", "width")}},
-        "plain_gteps": round(gteps_plain, 3),
-        "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain > 0 else None,
+        "plain_gteps": round(gteps_plain, 3) if gteps_plain else None,
+        "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain else None,
         "model_bytes_gbs": round(st["bytes"] / t_batched / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE)",
-                     "kernel": f"k_ms_{args.kernel}",
+                     "kernel": f"k_ms_{args.kernel}" if kt else None,
                      "avg_launch_us": round(k_ms / max(k_launch, 1) * 1e3, 2)},
         "cpu_baseline": cpu,
         "serving": serving,

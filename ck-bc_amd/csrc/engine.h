@@ -74,7 +74,8 @@ struct DevGraph {
   std::vector<int32_t> h_round;  // removal round per vertex on the host
   std::vector<uint32_t> h_sig;   // batching key: BFS depths from the four largest hubs
   std::vector<int32_t> h_topnb;  // batching key: highest-degree neighbour
-  int depth_hint = 0;            // deepest level of the hubs' BFS (capped at 254)
+  int depth_hint = 0;            // deepest level of the hubs' BFS
+  int cc_passes = 0;             // union-find passes the component labels took
   // Vertices are relabelled by descending degree on upload (hubs' rows and
   // masks then share cache lines); the caller's ids are mapped at the ABI.
   bool relabeled = false;

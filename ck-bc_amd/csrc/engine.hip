@@ -438,7 +438,9 @@ __device__ __forceinline__ bool decide_common(Ctl* c, int64_t* rec_lo, int64_t a
   c->started = 1;
   int d = c->level;
   rec_lo[d + 1] = c->rec_count;
-  if (rec_lo[d + 1] == rec_lo[d] || c->overflow) {
+  // stop early when the work list overflowed (retried with more room) or an
+  // exact path count wrapped (the sweep is discarded or redone scaled)
+  if (rec_lo[d + 1] == rec_lo[d] || c->overflow || (c->sigma_ovf && !c->scaled)) {
     c->fdone = 1;
     c->last = d - 1;
     return false;
@@ -1253,10 +1255,33 @@ __global__ void k_cc_union(const int64_t* off, const int32_t* adj, const int32_t
     }
   }
 }
-__global__ void k_cc_finish(int32_t* par, int64_t n, const int64_t* coff, int64_t* csize, int64_t* carcs) {
+// Labels from the union-find forest (par keeps its links; only atomic
+// operations touch it).
+__global__ void k_cc_label(int32_t* par, int64_t n, int32_t* label) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    label[v] = uf_find(par, (int32_t)v);
+}
+// Flags an arc whose endpoints carry different labels; the host then runs
+// another union and labelling pass, until every arc agrees.
+__global__ void k_cc_check(const int64_t* off, const int32_t* adj, const int32_t* rv, const int32_t* rj,
+                           int64_t nrec, int chunk, const int32_t* label, int* bad) {
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  int any = 0;
+  for (int64_t r = gw; r < nrec; r += tw) {
+    int32_t v = rv[r];
+    int j = rj[r];
+    int64_t e0 = off[v] + (int64_t)j * chunk;
+    int64_t e1 = min(off[v + 1], e0 + chunk);
+    const int32_t lv = label[v];
+    for (int64_t e = e0 + lane; e < e1; e += 64) any |= label[adj[e]] != lv;
+  }
+  if (__ballot(any) && lane_id() == 0) atomicOr(bad, 1);
+}
+__global__ void k_cc_sizes(const int32_t* label, int64_t n, const int64_t* coff, int64_t* csize, int64_t* carcs) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-    int32_t l = uf_find(par, (int32_t)v);
-    par[v] = l;
+    int32_t l = label[v];
     atomicAdd((u64*)&csize[l], 1ull);
     int64_t dg = coff[v + 1] - coff[v];
     if (dg) atomicAdd((u64*)&carcs[l], (u64)dg);
@@ -1448,19 +1473,27 @@ static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std
   std::vector<std::thread> th;
   for (int h = 0; h < (int)hubs.size(); h++)
     th.emplace_back([&, h] {
+      // level-synchronous BFS: depths are stored saturated at 254 (255 =
+      // unreached), the level count stays exact
       std::vector<uint8_t>& D = dep[h];
       std::vector<int32_t> q;
       q.reserve(1024);
       q.push_back(hubs[h]);
       D[hubs[h]] = 0;
-      for (size_t i = 0; i < q.size(); i++) {
-        int32_t u = q[i];
-        uint8_t du = D[u];
-        if (du >= 254) continue;
-        for (int64_t e = off[u]; e < off[u + 1]; e++)
-          if (D[adj[e]] == 255) { D[adj[e]] = du + 1; q.push_back(adj[e]); }
+      size_t lo = 0;
+      int level = 0;
+      while (lo < q.size()) {
+        size_t hi = q.size();
+        uint8_t dn = (uint8_t)std::min(level + 1, 254);
+        for (size_t i = lo; i < hi; i++) {
+          int32_t u = q[i];
+          for (int64_t e = off[u]; e < off[u + 1]; e++)
+            if (D[adj[e]] == 255) { D[adj[e]] = dn; q.push_back(adj[e]); }
+        }
+        lo = hi;
+        if (q.size() > hi) level++;
       }
-      hmax[h] = D[q.back()];
+      hmax[h] = level;
     });
   for (auto& t : th) t.join();
   *max_depth = 0;
@@ -1646,12 +1679,35 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
       HIP_TRY(hipMemcpyAsync(dv, fv.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
       HIP_TRY(hipMemcpyAsync(dj, fj.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
     }
-    hipLaunchKernelGGL(k_iota, G, B, 0, st, g.label, n);
+    // Connected components: lock-free union-find, labels = smallest id of
+    // each component, verified against every arc (and redone until they agree).
+    int32_t* par = nullptr;
+    int* bad = nullptr;
+    int* hbad = nullptr;
+    TRY(dalloc(&par, n));
+    TRY(dalloc(&bad, 1));
+    HIP_TRY(hipHostMalloc((void**)&hbad, sizeof(int), hipHostMallocDefault));
+    hipLaunchKernelGGL(k_iota, G, B, 0, st, par, n);
     hipLaunchKernelGGL(k_fill_u64, G, B, 0, st, (u64*)g.comp_size, 0ull, n);
     hipLaunchKernelGGL(k_fill_u64, G, B, 0, st, (u64*)g.comp_arcs, 0ull, n);
-    if (nf) hipLaunchKernelGGL(k_cc_union, 2048, 256, 0, st, g.foff, g.fadj, dv, dj, nf, prm.chunk, g.label);
-    hipLaunchKernelGGL(k_cc_finish, G, B, 0, st, g.label, n, g.off, g.comp_size, g.comp_arcs);
+    int pass = 0;
+    for (;; pass++) {
+      if (nf) hipLaunchKernelGGL(k_cc_union, 2048, 256, 0, st, g.foff, g.fadj, dv, dj, nf, prm.chunk, par);
+      hipLaunchKernelGGL(k_cc_label, G, B, 0, st, par, n, g.label);
+      hipLaunchKernelGGL(k_fill_i32, 1, 64, 0, st, bad, 0, (int64_t)1);
+      if (nf) hipLaunchKernelGGL(k_cc_check, 2048, 256, 0, st, g.foff, g.fadj, dv, dj, nf, prm.chunk, g.label, bad);
+      HIP_TRY(hipMemcpyAsync(hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      if (!*hbad) break;
+      if (pass >= 64) {
+        (void)hipFree(par); (void)hipFree(bad); (void)hipHostFree(hbad);
+        return set_error(BCMI_ERR_HIP, "component labelling did not converge");
+      }
+    }
+    hipLaunchKernelGGL(k_cc_sizes, G, B, 0, st, g.label, n, g.off, g.comp_size, g.comp_arcs);
     HIP_TRY(hipStreamSynchronize(st));
+    g.cc_passes = pass + 1;
+    (void)hipFree(par); (void)hipFree(bad); (void)hipHostFree(hbad);
     (void)hipFree(dv); (void)hipFree(dj);
   }
   HIP_TRY(hipGetLastError());
@@ -1872,7 +1928,9 @@ static int ws_init(Workspace* ws, const DevGraph& g, int mode, int width, const 
   } else {
     TRY(dalloc(&ws->seen, n, acc));
     TRY(dalloc(&ws->lm, 3 * n, acc));
-    ws->lscale_cap = std::min<int64_t>(n + 2, 4096);
+    // a sweep has at most 2 x (hub BFS depth) + 1 levels within the hubs'
+    // component; deeper components elsewhere grow the table on demand
+    ws->lscale_cap = std::min<int64_t>(n + 2, std::max<int64_t>(4096, 2 * (int64_t)g.depth_hint + 8));
     TRY(dalloc(&ws->lscale, (size_t)ws->lscale_cap * 64, acc));
     TRY(dalloc(&ws->lvmax, 64, acc));
     ws->nw = (n + 31) / 32;

@@ -344,18 +344,41 @@ int gen_block_chain(int64_t nblocks, int block, double avgdeg, uint64_t seed, bc
   int64_t per = (int64_t)(block * avgdeg / 2.0 + 0.5);
   int64_t k = nblocks * per + (nblocks - 1);
   std::vector<int32_t> T(k), H(k);
+  // Bridge endpoints come from the largest connected part of their block
+  // (an Erdos-Renyi block can leave a few vertices isolated), so the chain
+  // is one component: entry[bi] joins the previous block, exit[bi] the next.
+  std::vector<int32_t> entry(nblocks), exitv(nblocks);
 #pragma omp parallel for schedule(static)
   for (int64_t bi = 0; bi < nblocks; bi++) {
     Rng r(splitmix64(seed) ^ splitmix64((uint64_t)bi + 0x1234567ull));
     int64_t base = bi * block;
+    std::vector<int32_t> par(block), cnt(block, 0);
+    for (int i = 0; i < block; i++) par[i] = i;
+    auto find = [&](int x) {
+      while (par[x] != x) x = par[x] = par[par[x]];
+      return x;
+    };
     for (int64_t e = 0; e < per; e++) {
-      T[bi * per + e] = (int32_t)(base + (int64_t)r.below(block));
-      H[bi * per + e] = (int32_t)(base + (int64_t)r.below(block));
+      int a = (int)r.below(block), b = (int)r.below(block);
+      T[bi * per + e] = (int32_t)(base + a);
+      H[bi * per + e] = (int32_t)(base + b);
+      int ra = find(a), rb = find(b);
+      if (ra != rb) par[ra] = rb;
     }
-    if (bi + 1 < nblocks) {
-      T[nblocks * per + bi] = (int32_t)(base + (int64_t)r.below(block));
-      H[nblocks * per + bi] = (int32_t)(base + block + (int64_t)r.below(block));
+    int best = 0;
+    for (int i = 0; i < block; i++) {
+      int q = find(i);
+      if (++cnt[q] > cnt[best]) best = q;
     }
+    std::vector<int32_t> big;
+    for (int i = 0; i < block; i++)
+      if (find(i) == best) big.push_back(i);
+    entry[bi] = (int32_t)(base + big[r.below(big.size())]);
+    exitv[bi] = (int32_t)(base + big[r.below(big.size())]);
+  }
+  for (int64_t bi = 0; bi + 1 < nblocks; bi++) {
+    T[nblocks * per + bi] = exitv[bi];
+    H[nblocks * per + bi] = entry[bi + 1];
   }
   bcmi_graph* g = csr_from_arcs(n, T.data(), H.data(), k, 1, 0);
   if (!g) return set_error(BCMI_ERR_NOMEM, "out of host memory");

@@ -553,4 +553,45 @@ double orc_bytes_plain(int64_t n_s, int64_t m_s, int64_t a_s) {
     return 16.0 * (double)m_s + 24.0 * (double)a_s + 64.0 * (double)n_s;
 }
 
+/* Distance checksum per source: out[i] = sum over vertices t reached from
+ * src[i], t != src[i], of (d(src[i], t) - 1).  It equals the sum over v of
+ * the dependency delta_s(v) (every shortest s-t path has d(s,t) - 1 inner
+ * vertices and the path fractions of each t sum to one), so it checks a BC
+ * run without path counts - also where they exceed 2^64.  Plain BFS,
+ * OpenMP over sources. */
+int orc_depth_sums(int64_t n, const int64_t *off, const int32_t *adj, const int32_t *src, int64_t nsrc,
+                   int nthreads, int64_t *out) {
+    int err = 0;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        int32_t *depth = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+        int32_t *q = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+        if (!depth || !q) {
+#pragma omp atomic write
+            err = 1;
+        } else {
+            for (int64_t v = 0; v < n; v++) depth[v] = -1;
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t i = 0; i < nsrc; i++) {
+                int32_t s = src[i];
+                int64_t nq = 0, sum = 0;
+                q[nq++] = s;
+                depth[s] = 0;
+                for (int64_t h = 0; h < nq; h++) {
+                    int32_t c = q[h];
+                    for (int64_t k = off[c]; k < off[c + 1]; k++) {
+                        int32_t r = adj[k];
+                        if (depth[r] == -1) { depth[r] = depth[c] + 1; q[nq++] = r; sum += depth[r] - 1; }
+                    }
+                }
+                for (int64_t h = 0; h < nq; h++) depth[q[h]] = -1;
+                out[i] = sum;
+            }
+        }
+        free(depth);
+        free(q);
+    }
+    return err ? -ORC_ERR_NOMEM : 0;
+}
+
 int orc_version(void) { return 1; }

@@ -54,6 +54,8 @@ def lib():
         L.orc_bc_pipeline.restype = ctypes.c_int
         L.orc_bc_pipeline.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p, ctypes.c_int64,
                                       ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int]
+        L.orc_depth_sums.restype = ctypes.c_int
+        L.orc_depth_sums.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p, ctypes.c_int64, ctypes.c_int, _i64p]
         _LIB = L
     return _LIB
 
@@ -175,6 +177,18 @@ def bc_pipeline(off, adj, sources, heuristic=0, expand=False, nthreads=1):
     if rc != 0:
         raise OracleError(f"orc_bc_pipeline rc={rc}")
     return bc
+
+
+def depth_sums(off, adj, sources, nthreads=1):
+    """Per source: sum over reached t != s of (d(s,t) - 1), which equals the
+    sum of that source's dependencies delta_s(v) over all v."""
+    n = off.shape[0] - 1
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    out = np.zeros(len(src), np.int64)
+    rc = lib().orc_depth_sums(n, off, adj, src, len(src), nthreads, out)
+    if rc != 0:
+        raise OracleError(f"orc_depth_sums rc={rc}")
+    return out
 
 
 def brandes_highprec(off, adj, sources, digits=50):

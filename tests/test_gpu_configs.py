@@ -1,0 +1,124 @@
+"""Parity at the full sizes of the larger workloads, through checks whose
+cost does not grow with the number of sources: the pruned set and weights
+against the oracle, peeled vs unpeeled runs, plain vs batched sweeps, the
+CPU oracle on a few sources, and the distance checksum (the BC sum of a
+source equals the sum over its reached targets of d(s,t) - 1), which holds
+whatever the size of the path counts.
+
+  * R-MAT scale 18 + 30% pendant trees (n = 340,787), 8,192 sources;
+  * R-MAT scale 25 (33.5M vertices, ~1.07G arcs);
+  * a bridged-block chain of 32,768 blocks of 256 vertices (8.4M vertices,
+    path counts far beyond 2^64, tens of thousands of levels).
+"""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import bcmi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+SEED = 20250101
+NTH = min(16, os.cpu_count() or 1)
+
+
+def say(msg):
+    print(f"[configs] {msg}", file=sys.stderr, flush=True)
+
+
+def rel_sum(a, b):
+    return abs(float(a) - float(b)) / max(1.0, abs(float(b)))
+
+
+def test_pendant_trees_scale18():
+    t = time.time()
+    g = bcmi.Graph.rmat(18, 16, seed=SEED).with_pendant_trees(0.3, seed=SEED + 2)
+    src = g.sample_sources(8192, seed=SEED + 1)
+    say(f"pendant18: n={g.n} arcs={g.m} built in {time.time() - t:.1f}s")
+    e = bcmi.Engine()
+    e.upload(g, heuristic=bcmi.H_PEEL)
+    p = e.peel_info()
+    o = O.peel(g.off, g.adj)
+    assert np.array_equal(p["round"], o["round"])
+    assert np.array_equal(p["w"], o["w"])
+    rem = o["round"] > 0
+    assert rem.sum() > 0.2 * g.n
+    assert np.array_equal(p["parent"][rem], o["parent"][rem])
+    e.reset()
+    e.run(src, mode=bcmi.MODE_BATCHED)
+    bc_peel = e.bc()
+    say("pendant18: peeled batched run done")
+    # the same sources without peeling
+    u = bcmi.Engine()
+    u.upload(g)
+    u.reset()
+    u.run(src, mode=bcmi.MODE_BATCHED)
+    assert bcmi.rel_err(bc_peel, u.bc()) <= 1e-9
+    u.close()
+    # distance checksum over all 8,192 sources
+    ds = O.depth_sums(g.off, g.adj, src, nthreads=NTH)
+    assert rel_sum(bc_peel.sum(), ds.sum()) <= 1e-9
+    say("pendant18: checksum done")
+    # plain sweep with peeling vs the CPU pipeline on 256 sources
+    sub = src[:256]
+    e.reset()
+    e.run(sub, mode=bcmi.MODE_PLAIN)
+    ref = O.bc_pipeline(g.off, g.adj, sub, heuristic=1, nthreads=NTH)
+    assert bcmi.rel_err(e.bc(), ref) <= 1e-9
+    e.close()
+
+
+def test_rmat_scale25():
+    t = time.time()
+    g = bcmi.Graph.rmat(25, 16, seed=SEED)
+    src = g.sample_sources(4096, seed=SEED + 1)
+    say(f"rmat25: n={g.n} arcs={g.m} built in {time.time() - t:.1f}s")
+    assert g.m > 1_000_000_000
+    e = bcmi.Engine()
+    t = time.time()
+    e.upload(g)
+    say(f"rmat25: uploaded in {time.time() - t:.1f}s")
+    sub = src[:64]
+    e.reset()
+    e.run(sub, mode=bcmi.MODE_BATCHED)
+    bc = e.bc()
+    ds = O.depth_sums(g.off, g.adj, sub, nthreads=NTH)
+    assert rel_sum(bc.sum(), ds.sum()) <= 1e-9
+    say("rmat25: batched 64 sources + checksum done")
+    # batched vs plain vs the CPU oracle on two sources
+    two = src[:2]
+    e.reset()
+    e.run(two, mode=bcmi.MODE_BATCHED)
+    bb = e.bc()
+    e.reset()
+    e.run(two, mode=bcmi.MODE_PLAIN)
+    bp = e.bc()
+    ref = O.brandes(g.off, g.adj, two, nthreads=2)
+    assert bcmi.rel_err(bb, ref) <= 1e-9
+    assert bcmi.rel_err(bp, ref) <= 1e-9
+    say("rmat25: two sources vs oracle done")
+    e.close()
+
+
+def test_chain_32768_blocks():
+    t = time.time()
+    g = bcmi.Graph.block_chain(32768, 256, 8.0, seed=SEED)
+    src = g.sample_sources(1024, seed=SEED + 1)
+    say(f"chain32768: n={g.n} arcs={g.m} built in {time.time() - t:.1f}s")
+    e = bcmi.Engine()
+    e.upload(g)
+    sub = src[:256]
+    e.reset()
+    t = time.time()
+    st = e.run(sub, mode=bcmi.MODE_BATCHED)
+    say(f"chain32768: 256 sources in {time.time() - t:.1f}s, {st['max_levels']} levels")
+    assert st["max_levels"] > 30000
+    assert st["scaled_batches"] == st["batches"]       # path counts beyond 2^64
+    bc = e.bc()
+    assert np.all(np.isfinite(bc)) and np.all(bc >= 0)
+    ds = O.depth_sums(g.off, g.adj, sub, nthreads=NTH)
+    assert rel_sum(bc.sum(), ds.sum()) <= 1e-9
+    e.close()

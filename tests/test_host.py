@@ -128,6 +128,11 @@ def test_block_chain_generator():
     cross = (src // 64) != (g.adj // 64)
     assert int(cross.sum()) == 2 * 9                    # one bridge between consecutive blocks
     assert np.all(np.abs(src[cross] // 64 - g.adj[cross] // 64) == 1)
+    lab, _ = O.components(g.off, g.adj)                  # bridges join the blocks' large parts
+    big = np.bincount(lab).max()
+    assert big >= 0.99 * g.n
+    for a, b in zip(src[cross], g.adj[cross]):
+        assert lab[a] == lab[b] and np.sum(lab == lab[a]) == big
 
 
 def test_sample_sources():

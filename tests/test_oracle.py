@@ -282,3 +282,28 @@ def test_highprec_oracle_matches_uint64_oracle():
     # in a diamond chain every middle vertex of a diamond carries half of the
     # paths to the vertices after it
     assert bc[1] == bc[2] and bc[1] > 0
+
+
+def test_depth_sums_closed_forms_and_bc_identity():
+    """Distance checksum: on a path from an end it is (n-1)(n-2)/2; from the
+    centre of a star 0; on any graph it equals the sum of the sources'
+    dependencies (the BC sum), here checked on random graphs, and beyond
+    2^64 path counts against the big-integer Brandes."""
+    off, adj = csr(*G.path(12))
+    assert O.depth_sums(off, adj, [0])[0] == 11 * 10 // 2
+    assert O.depth_sums(off, adj, [5])[0] == sum(k - 1 for k in range(1, 6)) + sum(k - 1 for k in range(1, 7))
+    off, adj = csr(*G.star(9))
+    assert O.depth_sums(off, adj, [0])[0] == 0
+    assert O.depth_sums(off, adj, [1])[0] == 9 - 1          # the 8 other leaves at distance 2
+    for seed in range(4):
+        e, n = G.random_graph(80, 200, seed)
+        off, adj = csr(e, n)
+        src = np.arange(0, n, 7)
+        bc = O.brandes(off, adj, src)
+        ds = O.depth_sums(off, adj, src, nthreads=2)
+        assert abs(bc.sum() - ds.sum()) <= 1e-9 * max(1.0, ds.sum())
+        for s, d in zip(src[:3], ds[:3]):                   # per source
+            assert abs(O.brandes(off, adj, [s]).sum() - d) <= 1e-9 * max(1.0, d)
+    off, adj = csr(*G.diamonds(70))
+    hp = O.brandes_highprec(off, adj, [0, 5])
+    assert abs(hp.sum() - O.depth_sums(off, adj, [0, 5]).sum()) <= 1e-9 * hp.sum()
