@@ -18,7 +18,7 @@ and a served-query run (p50/p99 latency).
 Other workloads (not the default line):
   --scale 18 --pendant 0.3 --sources 8192     R-MAT 18 + 30% pendant trees, peeled
   --scale 25 --sources 4096 --steps 62        R-MAT scale 25
-  --graph chain --chain-blocks 32768 --sources 1024 --steps 12 --warmup 4 --kernel none
+  --graph chain --chain-blocks 32768 --sources 1024 --steps 12 --warmup 4 --kernel none --wg-mem 96
                                               bridged-block chain, 8.4M vertices
 """
 import argparse
@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--graph", default="rmat", choices=["rmat", "chain"],
                     help="chain: bridged-block chain of --chain-blocks Erdos-Renyi blocks of 256 vertices, degree 8")
     ap.add_argument("--chain-blocks", type=int, default=32768)
+    ap.add_argument("--wg-mem", type=int, default=0, metavar="GIB",
+                    help="device memory for one-workgroup-per-source sweeps on deep graphs (0: engine default)")
     ap.add_argument("--pendant", type=float, default=0.0,
                     help="attach pendant trees adding this fraction of vertices and peel them (-H 1, exact sources)")
     ap.add_argument("--serve-queries", type=int, default=48, help="0 skips the serving run")
@@ -178,6 +180,8 @@ def main():
     src = g.sample_sources(args.sources, seed=args.seed + 1)
 
     e = bcmi.Engine(0)
+    if args.wg_mem:
+        e.set_param("wg_mem", args.wg_mem << 30)
     import autotune
     tuned = None
     if args.graph == "rmat":

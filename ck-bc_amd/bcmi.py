@@ -51,7 +51,7 @@ class RunStats(ctypes.Structure):
                 ("bytes", ctypes.c_double), ("gpu_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("max_levels", ctypes.c_int64), ("sum_n", ctypes.c_int64), ("sum_m", ctypes.c_int64),
                 ("sum_a", ctypes.c_int64), ("sum_pairs", ctypes.c_int64), ("sum_union", ctypes.c_int64),
-                ("scaled_batches", ctypes.c_int64)]
+                ("scaled_batches", ctypes.c_int64), ("wg_sources", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -124,6 +124,18 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "sparse_verts") {
     if (v < 0) return set_error(BCMI_ERR_ARG, "sparse_verts must be >= 0");
     p.sparse_verts = v;
+  } else if (k == "strategy") {
+    if (v < 0 || v > 2) return set_error(BCMI_ERR_ARG, "strategy must be 0 (auto), 1 (level loop) or 2 (workgroup per source)");
+    p.strategy = (int)v;
+  } else if (k == "wg_depth") {
+    if (v < 1) return set_error(BCMI_ERR_ARG, "wg_depth must be >= 1");
+    p.wg_depth = (int)std::min<int64_t>(v, INT32_MAX);
+  } else if (k == "wg_mem") {
+    if (v < (1 << 20)) return set_error(BCMI_ERR_ARG, "wg_mem must be >= 1 MiB");
+    p.wg_mem = v;
+  } else if (k == "wg_slots") {
+    if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "wg_slots must be in [1,65536]");
+    p.wg_slots = (int)v;
   } else if (k == "streams") {
     if (v < 1 || v > 16) return set_error(BCMI_ERR_ARG, "streams must be in [1,16]");
     p.streams = (int)v;
@@ -171,6 +183,10 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "sgrid") *v = p.sgrid;
   else if (k == "sparse_verts") *v = p.sparse_verts;
   else if (k == "streams") *v = p.streams;
+  else if (k == "strategy") *v = p.strategy;
+  else if (k == "wg_depth") *v = p.wg_depth;
+  else if (k == "wg_mem") *v = p.wg_mem;
+  else if (k == "wg_slots") *v = p.wg_slots;
   else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
   return BCMI_OK;
 }
@@ -275,6 +291,7 @@ int bcmi_bc_run(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int mode, int
     st->sum_a = R.sum_a;
     st->sum_pairs = R.pairs;
     st->scaled_batches = R.scaled_batches;
+    st->wg_sources = R.wg_sources;
     for (int64_t u : R.uni_n) st->sum_union += u;
   }
   return rc;

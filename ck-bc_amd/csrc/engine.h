@@ -110,7 +110,8 @@ struct RunResult {
   int64_t sum_n = 0, sum_m = 0, sum_a = 0;
   int64_t n_td = 0, n_bu = 0;
   int64_t pairs = 0;                  // batched: (vertex, depth) pairs
-  int64_t scaled_batches = 0;         // batches recomputed with scaled FP64 path counts
+  int64_t scaled_batches = 0;         // batches (or workgroup sweeps) run with scaled FP64 path counts
+  int64_t wg_sources = 0;             // sources run one per workgroup (deep graphs)
   std::vector<int64_t> uni_n, uni_m;  // batched: union reach per batch
   double te = 0, bytes = 0, gpu_ms = 0, wall_ms = 0;
   double ktime_ms = 0;      // summed duration of the timed kernel (Params::ktime)

@@ -1206,6 +1206,208 @@ __global__ void k_ms_cleanup(GA g, BA b) {
   }
 }
 
+// ====================================== workgroup-per-source sweeps
+// Deep, narrow graphs (long chains) give every level a few hundred vertices
+// per source: a device-wide level loop then pays kernel boundaries on tens
+// of thousands of levels.  Here one workgroup runs a whole Brandes sweep for
+// one source (levels separated by workgroup barriers, not kernels) and
+// hundreds of workgroups run different sources at once.  Per-slot arrays are
+// private to their workgroup; within it they are accessed with
+// workgroup-scope atomics (loads served by the L2, which the RMWs update).
+#define WGT 256
+struct WA {
+  const int32_t* src;
+  const double* mult;
+  int64_t nsrc;
+  unsigned long long* next;  // next source index
+  int32_t* depth;            // slot-major, n per slot; -1 between sweeps
+  u64* sigma;                // uint64 counts, or FP64 bits when scaled; 0 between sweeps
+  double* delta;
+  int32_t* queue;            // BFS order; level k = queue[lvl[k] .. lvl[k+1])
+  int32_t* lvl;              // lcap per slot
+  double* scale;             // scaled: per-level factor m(k), lcap per slot
+  int64_t lcap;
+  double* bc;
+  int64_t* st_n;             // per source: reached vertices, their degree sum, DAG arcs
+  int64_t* st_m;
+  int64_t* st_a;
+  int32_t* st_levels;
+  int32_t* st_flag;          // 1: exact path count wrapped, 2: more levels than lcap
+  int scaled;
+};
+
+template <class T>
+__device__ __forceinline__ T wg_ld(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ void wg_st(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ double ub2d(u64 x) { return __longlong_as_double((long long)x); }
+__device__ __forceinline__ u64 d2ub(double x) { return (u64)__double_as_longlong(x); }
+
+__device__ __forceinline__ double block_max_d(double x, double* red) {
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  if (lane_id() == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  double m = red[0];
+  for (int i = 1; i < WGT / 64; i++) m = fmax(m, red[i]);
+  __syncthreads();
+  return m;
+}
+__device__ __forceinline__ long long block_sum_ll(long long x, long long* red) {
+  x = wave_sum_ll(x);
+  if (lane_id() == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  long long t = 0;
+  for (int i = 0; i < WGT / 64; i++) t += red[i];
+  __syncthreads();
+  return t;
+}
+
+__global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
+  const int64_t n = g.n;
+  const int tid = threadIdx.x;
+  const int64_t slot = blockIdx.x;
+  int32_t* depth = a.depth + slot * n;
+  u64* sig = a.sigma + slot * n;
+  double* del = a.delta + slot * n;
+  int32_t* q = a.queue + slot * n;
+  int32_t* lvl = a.lvl + slot * a.lcap;
+  double* scl = a.scale + slot * a.lcap;
+  __shared__ int64_t s_i;
+  __shared__ int s_tail, s_flag;
+  __shared__ double s_redd[WGT / 64];
+  __shared__ long long s_redl[WGT / 64];
+  const bool sc = a.scaled != 0;
+  for (;;) {
+    if (tid == 0) s_i = (int64_t)atomicAdd(a.next, 1ull);
+    __syncthreads();
+    const int64_t i = s_i;
+    if (i >= a.nsrc) break;
+    const int32_t s = a.src[i];
+    if (tid == 0) {
+      wg_st(depth + s, 0);
+      wg_st(sig + s, sc ? d2ub(1.0) : 1ull);
+      wg_st(del + s, 0.0);
+      wg_st(q, s);
+      wg_st(lvl, 0);
+      wg_st(lvl + 1, 1);
+      s_tail = 1;
+      s_flag = 0;
+    }
+    __syncthreads();
+    long long m_cnt = 0, a_cnt = 0;
+    // Forward: expand level L into level L+1 (first visit claims the vertex
+    // with a CAS on depth; sigma of level L+1 adds up its parents').
+    int L = 0;
+    for (;;) {
+      const int lo = wg_ld(lvl + L), hi = wg_ld(lvl + L + 1);
+      if (lo == hi) break;
+      if (L + 2 >= a.lcap) {
+        if (tid == 0) s_flag = 2;
+        __syncthreads();
+        break;
+      }
+      const int32_t d1 = L + 1;
+      for (int qq = lo + tid; qq < hi; qq += WGT) {
+        const int32_t v = wg_ld(q + qq);
+        const u64 sv = wg_ld(sig + v);
+        const int64_t e0 = g.off[v], e1 = g.off[v + 1];
+        m_cnt += e1 - e0;
+        for (int64_t e = e0; e < e1; e++) {
+          const int32_t w = g.adj[e];
+          int32_t dw = wg_ld(depth + w);
+          if (dw < 0) {
+            int32_t expect = -1;
+            if (__hip_atomic_compare_exchange_strong(depth + w, &expect, d1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP)) {
+              const int pos = atomicAdd(&s_tail, 1);
+              wg_st(q + pos, w);
+              wg_st(del + w, g.w ? (double)g.w[w] : 0.0);
+              dw = d1;
+            } else {
+              dw = expect;
+            }
+          }
+          if (dw == d1) {
+            a_cnt++;
+            if (sc) {
+              __hip_atomic_fetch_add((double*)(sig + w), ub2d(sv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+              u64 old = __hip_atomic_fetch_add(sig + w, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              if (old + sv < old) s_flag = 1;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      const int nt = s_tail;
+      if (sc && nt > hi) {
+        // factor of level L+1: its largest raw count; stored counts <= 1
+        double mx = 0.0;
+        for (int qq = hi + tid; qq < nt; qq += WGT) mx = fmax(mx, ub2d(wg_ld(sig + wg_ld(q + qq))));
+        const double m = block_max_d(mx, s_redd);
+        const double f = m > 0.0 ? m : 1.0;
+        for (int qq = hi + tid; qq < nt; qq += WGT) {
+          const int32_t w = wg_ld(q + qq);
+          wg_st(sig + w, d2ub(ub2d(wg_ld(sig + w)) / f));
+        }
+        if (tid == 0) wg_st(scl + d1, f);
+      }
+      if (tid == 0) wg_st(lvl + L + 2, nt);
+      __syncthreads();
+      if (s_flag) break;
+      L++;
+    }
+    const int reached = s_tail;
+    const int flag = s_flag;
+    // Backward, deepest level first: delta(v) = w(v) + sum over successors z
+    // of (delta(z)+1)/sigma(z)*sigma(v), in adjacency order (the oracle's).
+    if (!flag) {
+      const double mult = a.mult[i];
+      for (int d = L - 1; d >= 1; d--) {
+        const int lo = wg_ld(lvl + d), hi = wg_ld(lvl + d + 1);
+        const double inv_m = (sc && d + 1 < L) ? 1.0 / wg_ld(scl + d + 1) : 1.0;
+        for (int qq = lo + tid; qq < hi; qq += WGT) {
+          const int32_t v = wg_ld(q + qq);
+          const u64 sb = wg_ld(sig + v);
+          const double sv = sc ? ub2d(sb) * inv_m : (double)sb;
+          double acc = g.w ? (double)g.w[v] : 0.0;
+          for (int64_t e = g.off[v]; e < g.off[v + 1]; e++) {
+            const int32_t z = g.adj[e];
+            if (wg_ld(depth + z) == d + 1) {
+              const u64 zb = wg_ld(sig + z);
+              const double t = (wg_ld(del + z) + 1.0) / (sc ? ub2d(zb) : (double)zb);
+              acc = acc + t * sv;
+            }
+          }
+          wg_st(del + v, acc);
+          if (acc != 0.0) atomicAdd(a.bc + v, mult * acc);
+        }
+        __syncthreads();
+      }
+    }
+    const long long msum = block_sum_ll(m_cnt, s_redl);
+    const long long asum = block_sum_ll(a_cnt, s_redl);
+    if (tid == 0) {
+      a.st_n[i] = reached;
+      a.st_m[i] = msum;
+      a.st_a[i] = asum;
+      a.st_levels[i] = L;
+      a.st_flag[i] = flag;
+    }
+    // reset the visited entries for the next source of this slot
+    for (int qq = tid; qq < reached; qq += WGT) {
+      const int32_t v = wg_ld(q + qq);
+      wg_st(depth + v, -1);
+      wg_st(sig + v, 0ull);
+    }
+    __syncthreads();
+  }
+}
+
 // ================================================= graph preparation kernels
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -1231,8 +1433,10 @@ __device__ int32_t uf_find(int32_t* par, int32_t x) {
     x = gp;
   }
 }
+// Weak components: undirected lists hold each edge twice, so only the arc
+// to the larger id is used; directed lists use every arc.
 __global__ void k_cc_union(const int64_t* off, const int32_t* adj, const int32_t* rv, const int32_t* rj,
-                           int64_t nrec, int chunk, int32_t* par) {
+                           int64_t nrec, int chunk, int32_t* par, int directed) {
   const int lane = lane_id();
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
@@ -1243,7 +1447,7 @@ __global__ void k_cc_union(const int64_t* off, const int32_t* adj, const int32_t
     int64_t e1 = min(off[v + 1], e0 + chunk);
     for (int64_t e = e0 + lane; e < e1; e += 64) {
       int32_t x = adj[e];
-      if (x <= v) continue;
+      if (x == v || (x < v && !directed)) continue;
       int32_t a = v, bb = x;
       for (;;) {
         a = uf_find(par, a);
@@ -1692,7 +1896,7 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
     hipLaunchKernelGGL(k_fill_u64, G, B, 0, st, (u64*)g.comp_arcs, 0ull, n);
     int pass = 0;
     for (;; pass++) {
-      if (nf) hipLaunchKernelGGL(k_cc_union, 2048, 256, 0, st, g.foff, g.fadj, dv, dj, nf, prm.chunk, par);
+      if (nf) hipLaunchKernelGGL(k_cc_union, 2048, 256, 0, st, g.foff, g.fadj, dv, dj, nf, prm.chunk, par, directed);
       hipLaunchKernelGGL(k_cc_label, G, B, 0, st, par, n, g.label);
       hipLaunchKernelGGL(k_fill_i32, 1, 64, 0, st, bad, 0, (int64_t)1);
       if (nf) hipLaunchKernelGGL(k_cc_check, 2048, 256, 0, st, g.foff, g.fadj, dv, dj, nf, prm.chunk, g.label, bad);
@@ -1773,6 +1977,22 @@ struct Workspace {
   // and BC array, that run sweeps of the same call concurrently.
   std::vector<Workspace*> helpers;
   const DevGraph* helpers_graph = nullptr;
+  // workgroup-per-source sweeps (deep graphs): per-slot state and per-run buffers
+  int64_t wg_slots = 0, wg_lcap = 0;
+  int32_t* wg_depth = nullptr;
+  u64* wg_sigma = nullptr;
+  double* wg_delta = nullptr;
+  int32_t* wg_queue = nullptr;
+  int32_t* wg_lvl = nullptr;
+  double* wg_scale = nullptr;
+  double* wg_bc = nullptr;            // run result, folded into bc when the run succeeds
+  unsigned long long* wg_next = nullptr;
+  int64_t wg_cap = 0;                 // per-run buffers below hold wg_cap sources
+  int32_t* wg_src = nullptr;
+  double* wg_mult = nullptr;
+  int64_t* wg_st = nullptr;           // 3 x wg_cap: n, m, a
+  int32_t* wg_st2 = nullptr;          // 2 x wg_cap: levels, flag
+  size_t wg_bytes = 0;
   const DevGraph* cap_graph = nullptr;
   size_t bytes = 0;
 };
@@ -1823,16 +2043,20 @@ static int concurrent_sweeps(const DevGraph& g, const Params& prm) {
   return g.depth_hint >= 64 ? std::max(1, std::min(prm.streams, 16)) : 1;
 }
 
+static bool use_wg(const DevGraph& g, const Params& prm);
+static size_t wg_bytes_estimate(const DevGraph& g, const Params& prm);
+
 size_t workspace_bytes(const DevGraph& g, int mode, int width, const Params& prm) {
   (void)width;
-  const size_t copies = (size_t)concurrent_sweeps(g, prm);
+  const bool wg = use_wg(g, prm);
+  const size_t copies = wg ? 1 : (size_t)concurrent_sweeps(g, prm);
   int64_t n = g.n;
   int64_t cap = record_capacity(g, mode);
   size_t b = sizeof(double) * n + sizeof(int64_t) * (n + 3) + sizeof(int32_t) * n + 2 * sizeof(int32_t) * cap;
   b += 4 * sizeof(u64) * n;  // per-run source terms
   if (mode == BCMI_MODE_PLAIN) b += (sizeof(int32_t) + sizeof(u64) + 2 * sizeof(double)) * n;
   else b += sizeof(u64) * cap + (5 * sizeof(u64) + 64 * (sizeof(u64) + sizeof(double))) * n;
-  return b * copies;
+  return b * copies + (wg ? wg_bytes_estimate(g, prm) : 0);
 }
 
 static void free_graphs(Workspace* ws) {
@@ -1857,6 +2081,11 @@ void workspace_destroy(Workspace* ws) {
   for (Workspace* h : ws->helpers) workspace_destroy(h);
   ws->helpers.clear();
   free_graphs(ws);
+  {
+    auto fw = [](void* p) { if (p) (void)hipFree(p); };
+    fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_delta); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
+    fw(ws->wg_bc); fw(ws->wg_next); fw(ws->wg_src); fw(ws->wg_mult); fw(ws->wg_st); fw(ws->wg_st2);
+  }
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   if (ws->own_bc) fr(ws->bc);
   fr(ws->ctl); fr(ws->rec_v); fr(ws->rec_j); fr(ws->rec_mask); fr(ws->rec_lo); fr(ws->done);
@@ -1963,7 +2192,7 @@ Workspace* workspace_create(const DevGraph& g, int mode, int width, const Params
 }
 hipStream_t workspace_stream(Workspace* ws) { return ws->st; }
 size_t workspace_size(Workspace* ws) {
-  size_t b = ws->bytes;
+  size_t b = ws->bytes + ws->wg_bytes;
   for (Workspace* h : ws->helpers) b += h->bytes;
   return b;
 }
@@ -2323,6 +2552,158 @@ static int ensure_helpers(const DevGraph& g, Workspace* ws, const Params& prm, i
   return BCMI_OK;
 }
 
+// Workgroup-per-source sweeps serve deep graphs (hub BFS depth of at least
+// Params::wg_depth) unless Params::strategy forces one engine.
+static bool use_wg(const DevGraph& g, const Params& prm) {
+  if (prm.strategy == 1) return false;
+  if (prm.strategy == 2) return true;
+  return g.depth_hint >= prm.wg_depth;
+}
+
+static size_t wg_slot_bytes(int64_t n, int64_t lcap) {
+  return (size_t)n * (2 * sizeof(int32_t) + sizeof(u64) + sizeof(double)) +
+         (size_t)lcap * (sizeof(int32_t) + sizeof(double));
+}
+
+static int64_t wg_slot_count(int64_t n, const Params& prm, int64_t lcap) {
+  int64_t k = (int64_t)(prm.wg_mem / std::max<size_t>(wg_slot_bytes(n, lcap), 1));
+  return std::max<int64_t>(1, std::min<int64_t>(k, prm.wg_slots));
+}
+
+static int64_t wg_initial_lcap(const DevGraph& g) {
+  // a sweep inside the hubs' component has at most 2 x depth + 1 levels
+  return std::min<int64_t>(g.n + 2, std::max<int64_t>(1024, 2 * (int64_t)g.depth_hint + 64));
+}
+
+static size_t wg_bytes_estimate(const DevGraph& g, const Params& prm) {
+  const int64_t lcap = wg_initial_lcap(g);
+  return (size_t)wg_slot_count(g.n, prm, lcap) * wg_slot_bytes(g.n, lcap) + sizeof(double) * g.n + (1 << 20);
+}
+
+static int wg_alloc_slots(const DevGraph& g, Workspace* ws, const Params& prm, int64_t lcap) {
+  ALLOC_LOCK;
+  auto fw = [](void* p) { if (p) (void)hipFree(p); };
+  fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_delta); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
+  ws->wg_depth = nullptr; ws->wg_sigma = nullptr; ws->wg_delta = nullptr; ws->wg_queue = nullptr;
+  ws->wg_lvl = nullptr; ws->wg_scale = nullptr;
+  const int64_t n = g.n;
+  const int64_t S = wg_slot_count(n, prm, lcap);
+  size_t b = 0;
+  TRY(dalloc(&ws->wg_depth, (size_t)S * n, &b));
+  TRY(dalloc(&ws->wg_sigma, (size_t)S * n, &b));
+  TRY(dalloc(&ws->wg_delta, (size_t)S * n, &b));
+  TRY(dalloc(&ws->wg_queue, (size_t)S * n, &b));
+  TRY(dalloc(&ws->wg_lvl, (size_t)S * lcap, &b));
+  TRY(dalloc(&ws->wg_scale, (size_t)S * lcap, &b));
+  if (!ws->wg_bc) {
+    TRY(dalloc(&ws->wg_bc, n, &b));
+    TRY(dalloc(&ws->wg_next, 1, &b));
+    hipLaunchKernelGGL(k_fill_f64, grid_for(n), 256, 0, ws->st, ws->wg_bc, 0.0, n);
+  } else {
+    b += sizeof(double) * n + sizeof(unsigned long long);
+  }
+  hipLaunchKernelGGL(k_fill_i32, grid_for(S * n), 256, 0, ws->st, ws->wg_depth, -1, S * n);
+  hipLaunchKernelGGL(k_fill_u64, grid_for(S * n), 256, 0, ws->st, ws->wg_sigma, 0ull, S * n);
+  HIP_TRY(hipStreamSynchronize(ws->st));
+  ws->wg_slots = S;
+  ws->wg_lcap = lcap;
+  ws->wg_bytes = b + ws->wg_cap * (sizeof(int32_t) * 3 + sizeof(double) + sizeof(int64_t) * 3);
+  return BCMI_OK;
+}
+
+static int wg_alloc_run(Workspace* ws, int64_t k) {
+  if (k <= ws->wg_cap) return BCMI_OK;
+  ALLOC_LOCK;
+  auto fw = [](void* p) { if (p) (void)hipFree(p); };
+  fw(ws->wg_src); fw(ws->wg_mult); fw(ws->wg_st); fw(ws->wg_st2);
+  ws->wg_src = nullptr; ws->wg_mult = nullptr; ws->wg_st = nullptr; ws->wg_st2 = nullptr;
+  int64_t cap = std::max<int64_t>(k, 1024);
+  TRY(dalloc(&ws->wg_src, cap));
+  TRY(dalloc(&ws->wg_mult, cap));
+  TRY(dalloc(&ws->wg_st, 3 * cap));
+  TRY(dalloc(&ws->wg_st2, 2 * cap));
+  ws->wg_bytes += (cap - ws->wg_cap) * (sizeof(int32_t) * 3 + sizeof(double) + sizeof(int64_t) * 3);
+  ws->wg_cap = cap;
+  return BCMI_OK;
+}
+
+// Runs every source as its own sweep on the workgroup-per-source kernel.
+// BC goes to a scratch array first and is folded in only when the whole run
+// succeeded (an exact path count that wraps leaves the result untouched).
+static int run_wg(const DevGraph& g, Workspace* ws, const Params& prm, const std::vector<int32_t>& cs,
+                  const std::vector<double>& cm, RunResult& R, int& ovf) {
+  hipStream_t st = ws->st;
+  const int64_t lcap0 = std::max(wg_initial_lcap(g), ws->wg_lcap);
+  if (!ws->wg_depth || ws->wg_lcap < lcap0 || ws->wg_slots != wg_slot_count(g.n, prm, lcap0))
+    TRY(wg_alloc_slots(g, ws, prm, lcap0));
+  std::vector<int32_t> src(cs);
+  std::vector<double> mult(cm);
+  int scaled = prm.sigma == 1 || (prm.sigma == 2 && g.sigma_overflowed.load());
+  const GA ga = make_ga(g, prm);
+  for (int attempt = 0; !src.empty(); attempt++) {
+    if (attempt > 12) return set_error(BCMI_ERR_STATE, "workgroup sweeps did not complete");
+    const int64_t k = (int64_t)src.size();
+    TRY(wg_alloc_run(ws, k));
+    HIP_TRY(hipMemcpyAsync(ws->wg_src, src.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ws->wg_mult, mult.data(), sizeof(double) * k, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, st, (u64*)ws->wg_next, 0ull, (int64_t)1);
+    WA a;
+    a.src = ws->wg_src; a.mult = ws->wg_mult; a.nsrc = k; a.next = ws->wg_next;
+    a.depth = ws->wg_depth; a.sigma = ws->wg_sigma; a.delta = ws->wg_delta; a.queue = ws->wg_queue;
+    a.lvl = ws->wg_lvl; a.scale = ws->wg_scale; a.lcap = ws->wg_lcap; a.bc = ws->wg_bc;
+    a.st_n = ws->wg_st; a.st_m = ws->wg_st + ws->wg_cap; a.st_a = ws->wg_st + 2 * ws->wg_cap;
+    a.st_levels = ws->wg_st2; a.st_flag = ws->wg_st2 + ws->wg_cap;
+    a.scaled = scaled;
+    const int grid = (int)std::min<int64_t>(ws->wg_slots, k);
+    hipLaunchKernelGGL(k_wg_sweep, grid, WGT, 0, st, ga, a);
+    std::vector<int64_t> hn(k), hm(k), ha(k);
+    std::vector<int32_t> hl(k), hf(k);
+    HIP_TRY(hipMemcpyAsync(hn.data(), a.st_n, sizeof(int64_t) * k, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hm.data(), a.st_m, sizeof(int64_t) * k, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ha.data(), a.st_a, sizeof(int64_t) * k, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hl.data(), a.st_levels, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hf.data(), a.st_flag, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    R.launches++;
+    std::vector<int32_t> rs;
+    std::vector<double> rm;
+    bool wrapped = false, deep = false;
+    for (int64_t j = 0; j < k; j++) {
+      if (hf[j] == 0) {
+        R.n_s.push_back(hn[j]);
+        R.m_s.push_back(hm[j]);
+        R.a_s.push_back(ha[j]);
+        R.levels.push_back(hl[j]);
+        R.max_levels = std::max<int64_t>(R.max_levels, hl[j]);
+        R.sweeps++;
+        R.wg_sources++;
+        if (scaled) R.scaled_batches++;
+      } else {
+        rs.push_back(src[j]);
+        rm.push_back(mult[j]);
+        wrapped |= hf[j] == 1;
+        deep |= hf[j] == 2;
+      }
+    }
+    if (wrapped) {
+      if (prm.sigma == 0) {
+        ovf = 1;
+        hipLaunchKernelGGL(k_fill_f64, grid_for(g.n), 256, 0, st, ws->wg_bc, 0.0, g.n);
+        HIP_TRY(hipStreamSynchronize(st));
+        return BCMI_OK;
+      }
+      g.sigma_overflowed.store(true);
+      scaled = 1;
+    }
+    if (deep) TRY(wg_alloc_slots(g, ws, prm, std::min<int64_t>(g.n + 2, ws->wg_lcap * 4)));
+    src.swap(rs);
+    mult.swap(rm);
+  }
+  hipLaunchKernelGGL(k_add_take_f64, grid_for(g.n), 256, 0, st, ws->bc, ws->wg_bc, g.n);
+  return BCMI_OK;
+}
+
 int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* sources, int64_t nsrc, int mode,
            int src_flags, RunResult* out) {
   if (mode != ws->mode) return set_error(BCMI_ERR_STATE, "workspace was created for another mode");
@@ -2338,7 +2719,11 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   HIP_TRY(hipEventRecord(ws->ev0, st));
   const bool plain = mode == BCMI_MODE_PLAIN;
   const int W = std::max(1, std::min(64, prm.width));
-  if (!plain) {
+  const bool wg = use_wg(g, prm);
+  RunResult R;
+  int ovf = 0;
+  if (wg) TRY(run_wg(g, ws, prm, cs, cm, R, ovf));
+  if (!plain && !wg) {
     // Batch sources with similar depth profiles together (BC is a sum over
     // sources, so the order only changes the rounding of the final sums).
     std::vector<size_t> idx(cs.size());
@@ -2367,7 +2752,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
     cs.swap(cs2);
     cm.swap(cm2);
   }
-  const size_t nitems = plain ? cs.size() : (cs.size() + W - 1) / W;
+  const size_t nitems = wg ? 0 : plain ? cs.size() : (cs.size() + W - 1) / W;
   const int K = (int)std::min<size_t>((size_t)concurrent_sweeps(g, prm), std::max<size_t>(1, nitems));
   if (K > 1) TRY(ensure_helpers(g, ws, prm, K - 1));
   std::vector<RunPart> parts(K);
@@ -2409,8 +2794,6 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
     work(0);
     for (auto& x : th) x.join();
   }
-  RunResult R;
-  int ovf = 0;
   for (RunPart& P : parts) {
     if (P.rc != BCMI_OK) return set_error(P.rc, P.msg);
     RunResult& Q = P.R;

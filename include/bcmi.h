@@ -74,7 +74,8 @@ typedef struct bcmi_run_stats {
     int64_t sum_a;         /* sum of shortest-path DAG arcs */
     int64_t sum_pairs;     /* batched: (vertex, depth) pairs over batches, i.e. work-list vertices */
     int64_t sum_union;     /* batched: distinct vertices reached per batch, summed over batches */
-    int64_t scaled_batches; /* batches whose path counts exceeded 2^64-1 and were redone in scaled FP64 */
+    int64_t scaled_batches; /* batches (or workgroup sweeps) run with scaled FP64 path counts */
+    int64_t wg_sources;    /* sources run one per workgroup (deep graphs, see "strategy") */
 } bcmi_run_stats;
 
 /* ---- error reporting ---------------------------------------------------- */
@@ -123,7 +124,11 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * (blocks per kernel on small levels), "sparse_verts" (levels with fewer
  * vertices use the small grid), "streams" (sweeps of one call run at once,
  * each on its own stream and workspace, on graphs whose hub BFS depth is
- * 64 or more), "trace". */
+ * 64 or more), "strategy" (0 auto, 1 device-wide level loop, 2 one workgroup
+ * per source: deep graphs, whose hub BFS depth reaches "wg_depth", run
+ * hundreds of sources at once, one sweep per workgroup, within "wg_mem"
+ * bytes of per-source state and at most "wg_slots" sources in flight),
+ * "trace". */
 int bcmi_set_param(bcmi_ctx *ctx, const char *key, int64_t value);
 int bcmi_get_param(bcmi_ctx *ctx, const char *key, int64_t *value);
 

@@ -108,7 +108,7 @@ def test_chain_32768_blocks():
     g = bcmi.Graph.block_chain(32768, 256, 8.0, seed=SEED)
     src = g.sample_sources(1024, seed=SEED + 1)
     say(f"chain32768: n={g.n} arcs={g.m} built in {time.time() - t:.1f}s")
-    e = bcmi.Engine()
+    e = bcmi.Engine(wg_mem=48 << 30)
     e.upload(g)
     sub = src[:256]
     e.reset()
@@ -116,7 +116,8 @@ def test_chain_32768_blocks():
     st = e.run(sub, mode=bcmi.MODE_BATCHED)
     say(f"chain32768: 256 sources in {time.time() - t:.1f}s, {st['max_levels']} levels")
     assert st["max_levels"] > 30000
-    assert st["scaled_batches"] == st["batches"]       # path counts beyond 2^64
+    assert st["wg_sources"] == len(sub)                # deep graph: a workgroup per source
+    assert st["scaled_batches"] == len(sub)            # path counts beyond 2^64
     bc = e.bc()
     assert np.all(np.isfinite(bc)) and np.all(bc >= 0)
     ds = O.depth_sums(g.off, g.adj, sub, nthreads=NTH)

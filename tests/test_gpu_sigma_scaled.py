@@ -15,8 +15,8 @@ def test_chain_beyond_uint64():
     g = bcmi.Graph.block_chain(90, 128, 8.0, seed=6)
     src = np.array([0, 1, 5, g.n // 3, g.n // 2, g.n - 1], np.int32)
     ref = O.brandes_highprec(g.off, g.adj, src)
-    e = bcmi.Engine()
-    e.upload(g)
+    e = bcmi.Engine(strategy=1)       # the device-wide level loop (deep graphs default to
+    e.upload(g)                       # a workgroup per source, tested on its own)
     e.set_param("sigma", 0)
     e.reset()
     with pytest.raises(bcmi.BcmiError) as ex:
@@ -57,12 +57,16 @@ def test_scaled_equals_exact_without_overflow(peel):
 
 
 def test_long_path_many_levels_scale_table_growth():
-    """More levels than the initial scale table (4096) forces its growth."""
+    """A component deeper than the hubs' (the scale table starts from their
+    BFS depth, at least 4096 levels) forces the table to grow."""
     import graphs as G
-    g = bcmi.Graph.from_edges(*G.path(6000))
-    src = np.array([0, g.n - 1, 3000], np.int32)
+    r = bcmi.Graph.rmat(10, 16, seed=8)
+    pe, pn = G.path(6000)
+    edges = np.concatenate([np.stack([np.repeat(np.arange(r.n), np.diff(r.off)), r.adj], 1), pe + r.n])
+    g = bcmi.Graph.from_edges(edges, r.n + pn)
+    src = np.array([r.n, r.n + pn - 1, r.n + 3000, 5], np.int32)
     ref = O.brandes(g.off, g.adj, src)
-    e = bcmi.Engine()
+    e = bcmi.Engine(strategy=1)
     e.upload(g)
     e.set_param("sigma", 1)
     e.reset()
