@@ -299,7 +299,7 @@ def main():
         "config": {"workload": f"batched Brandes BC, {gdesc}, {len(timed)} sampled sources in batches of {width}",
                    "n": g.n, "arcs": g.m, "batch": width,
                    "tuned": {k: v for k, v in (tuned or {}).items() if k in
-                             ("chunk", "block", "grid", "alpha_batched", "sgrid", "width")}},
+                             ("chunk", "block", "grid", "alpha_batched", "width")}},
         "plain_gteps": round(gteps_plain, 3) if gteps_plain else None,
         "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain else None,
         "model_bytes_gbs": round(st["bytes"] / t_batched / 1e9, 1),
