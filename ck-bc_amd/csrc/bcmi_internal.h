@@ -42,7 +42,7 @@ struct Params {
                             // 2 highest-degree neighbour then signature
   int reorder = 1;          // relabel vertices by descending degree at upload (locality)
   int trace = 0;            // diagnostic: per-level statistics on stderr (use with steps = 1)
-  int ktime = 0;            // kernel timed with HIP events inside the level loop:
+  int ktime = 0;            // kernel timed on the device (wave entry/exit stamps) inside the level loop:
                             // 0 none, 1 batched sigma, 2 batched backward, 3 batched top-down,
                             // 4 batched bottom-up, 5 plain top-down, 6 plain bottom-up, 7 plain backward
 };

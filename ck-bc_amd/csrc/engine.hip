@@ -139,31 +139,47 @@ __device__ __forceinline__ int nchunks(int64_t deg, int chunk) {
   return deg <= chunk ? 1 : (int)((deg + chunk - 1) / chunk);
 }
 
-// Device-side kernel timer: first block entry to last block exit of every
-// launch, on the constant-rate wall clock (s_memrealtime), summed on device.
+// Device-side kernel timer on the constant-rate wall clock (s_memrealtime):
+// every wave of the timed kernel stamps its entry and exit into its own slot
+// (plain stores, no shared counters), and a one-block kernel launched right
+// after it adds (last exit - first entry) to the total.
 struct KT {
-  u64 start, end, total, launches;
-  unsigned count;
+  u64 total, launches;
+  u64* wstart;  // per wave of the launch
+  u64* wend;
 };
+#define KT_MAX_WAVES (65536 * 4)
+__device__ __forceinline__ int64_t kt_wave() {
+  return ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+}
 __device__ __forceinline__ void kt_enter(KT* kt) {
-  if (kt && threadIdx.x == 0) atomicMin(&kt->start, (u64)wall_clock64());
+  if (kt && __lane_id() == 0) kt->wstart[kt_wave()] = (u64)wall_clock64();
 }
 __device__ __forceinline__ void kt_exit(KT* kt) {
-  if (!kt) return;
+  if (kt && __lane_id() == 0) kt->wend[kt_wave()] = (u64)wall_clock64();
+}
+__global__ void __launch_bounds__(256) k_kt_accum(KT* kt, int64_t nwaves) {
+  __shared__ u64 s_lo[4], s_hi[4];
+  u64 lo = ~0ull, hi = 0;
+  for (int64_t i = threadIdx.x; i < nwaves; i += blockDim.x) {
+    u64 a = kt->wstart[i], b = kt->wend[i];
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    u64 a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if ((threadIdx.x & 63) == 0) { s_lo[threadIdx.x >> 6] = lo; s_hi[threadIdx.x >> 6] = hi; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    // Returning device-scope atomics complete at the coherence point before
-    // the next one issues, so the block that counts last sees every stamp.
-    u64 e_old = atomicMax(&kt->end, (u64)wall_clock64());
-    unsigned prev = atomicAdd(&kt->count, 1u + (unsigned)(e_old & 0));
-    if (prev == gridDim.x - 1) {
-      u64 s0 = ld_coherent_u64(&kt->start), e0 = ld_coherent_u64(&kt->end);
-      kt->total += e0 - s0;
-      kt->launches += 1;
-      kt->start = ~0ull;
-      kt->end = 0;
-      kt->count = 0;
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      lo = s_lo[w] < lo ? s_lo[w] : lo;
+      hi = s_hi[w] > hi ? s_hi[w] : hi;
     }
+    if (hi > lo) kt->total += hi - lo;
+    kt->launches += 1;
   }
 }
 
@@ -1973,6 +1989,8 @@ struct Workspace {
   Params cap_prm;
   KT* kt = nullptr;             // device timer of the kernel selected by Params::ktime
   KT* hkt = nullptr;            // pinned mirror
+  u64* kt_ws = nullptr;         // per-wave entry / exit stamps of the timed kernel
+  u64* kt_we = nullptr;
   // Deep graphs (long chains): further workspaces, each with its own stream
   // and BC array, that run sweeps of the same call concurrently.
   std::vector<Workspace*> helpers;
@@ -2067,12 +2085,11 @@ static void free_graphs(Workspace* ws) {
   ws->cap_graph = nullptr;
 }
 
-__global__ void k_kt_reset(KT* kt) {
-  kt->start = ~0ull;
-  kt->end = 0;
+__global__ void k_kt_reset(KT* kt, u64* ws, u64* we) {
   kt->total = 0;
   kt->launches = 0;
-  kt->count = 0;
+  kt->wstart = ws;
+  kt->wend = we;
 }
 
 void workspace_destroy(Workspace* ws) {
@@ -2095,6 +2112,8 @@ void workspace_destroy(Workspace* ws) {
   if (ws->hctl) (void)hipHostFree(ws->hctl);
   if (ws->hkt) (void)hipHostFree(ws->hkt);
   if (ws->kt) (void)hipFree(ws->kt);
+  if (ws->kt_ws) (void)hipFree(ws->kt_ws);
+  if (ws->kt_we) (void)hipFree(ws->kt_we);
   if (ws->hsrc) (void)hipHostFree(ws->hsrc);
   if (ws->hmult) (void)hipHostFree(ws->hmult);
   if (ws->hstats) (void)hipHostFree(ws->hstats);
@@ -2128,6 +2147,8 @@ static int ws_init(Workspace* ws, const DevGraph& g, int mode, int width, const 
   TRY(dalloc(&ws->ctl, 1, acc));
   HIP_TRY(hipHostMalloc((void**)&ws->hctl, sizeof(Ctl), hipHostMallocDefault));
   TRY(dalloc(&ws->kt, 1, acc));
+  TRY(dalloc(&ws->kt_ws, KT_MAX_WAVES, acc));
+  TRY(dalloc(&ws->kt_we, KT_MAX_WAVES, acc));
   HIP_TRY(hipHostMalloc((void**)&ws->hkt, sizeof(KT), hipHostMallocDefault));
   TRY(dalloc(&ws->rec_lo, n + 3, acc));
   TRY(dalloc(&ws->done, n, acc));
@@ -2143,7 +2164,7 @@ static int ws_init(Workspace* ws, const DevGraph& g, int mode, int width, const 
   hipStream_t st = ws->st;
   hipLaunchKernelGGL(k_fill_f64, G, 256, 0, st, ws->bc, 0.0, n);
   hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, ws->done, 0, n);
-  hipLaunchKernelGGL(k_kt_reset, 1, 1, 0, st, ws->kt);
+  hipLaunchKernelGGL(k_kt_reset, 1, 1, 0, st, ws->kt, ws->kt_ws, ws->kt_we);
   hipLaunchKernelGGL(k_fill_u64, grid_for(n + 3), 256, 0, st, (u64*)ws->rec_lo, 0ull, n + 3);
   if (mode == BCMI_MODE_PLAIN) {
     TRY(dalloc(&ws->depth, n, acc));
@@ -2227,6 +2248,10 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
   const bool plain = ws->mode == BCMI_MODE_PLAIN;
   KT* T = ws->kt;
   auto tk = [&](int id) -> KT* { return kt == id ? T : nullptr; };
+  // after a timed kernel of Gk blocks: fold its wave stamps into the total
+  auto ktacc = [&](int id, int Gk) {
+    if (kt == id) hipLaunchKernelGGL(k_kt_accum, 1, 256, 0, st, T, (int64_t)Gk * B / 64);
+  };
   // Four graphs: {large grid, small grid} x {forward, backward}.  The small
   // one serves levels with few vertices (long chains): its launches are cheap
   // and it never considers bottom-up (which scans every vertex).
@@ -2242,10 +2267,13 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
         if (fwd) {
           hipLaunchKernelGGL(k_plain_decide, 1, 1, 0, st, ga, pa, sparse ? 0 : 1);
           hipLaunchKernelGGL(k_plain_td, Gk, B, 0, st, ga, pa, tk(5));
+          ktacc(5, Gk);
           if (!sparse) hipLaunchKernelGGL(k_plain_bu, Gk, B, 0, st, ga, pa, tk(6));
+          if (!sparse) ktacc(6, Gk);
         } else {
           hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, pa.ctl);
           hipLaunchKernelGGL(k_plain_back, Gk, B, 0, st, ga, pa, tk(7));
+          ktacc(7, Gk);
           hipLaunchKernelGGL(k_plain_bfin, Gk, B, 0, st, ga, pa);
         }
       } else {
@@ -2253,11 +2281,15 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
         if (fwd) {
           hipLaunchKernelGGL(k_ms_decide, 1, 1, 0, st, ga, ba, sparse ? 0 : 1);
           hipLaunchKernelGGL(k_ms_td, Gk, B, 0, st, ga, ba, tk(3));
+          ktacc(3, Gk);
           if (!sparse) hipLaunchKernelGGL(k_ms_bu, Gk, B, 0, st, ga, ba, tk(4));
+          if (!sparse) ktacc(4, Gk);
           hipLaunchKernelGGL(k_ms_sigma, Gk, B, 0, st, ga, ba, tk(1));
+          ktacc(1, Gk);
         } else {
           hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, ba.ctl);
           hipLaunchKernelGGL(k_ms_back, Gk, B, 0, st, ga, ba, tk(2));
+          ktacc(2, Gk);
           hipLaunchKernelGGL(k_ms_bfin, Gk, B, 0, st, ga, ba);
         }
       }
@@ -2715,7 +2747,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   bool fin = false;
   TRY(prepare_sources(g, ws, sources, nsrc, src_flags, cs, cm, fin));
   hipStream_t st = ws->st;
-  hipLaunchKernelGGL(k_kt_reset, 1, 1, 0, st, ws->kt);
+  hipLaunchKernelGGL(k_kt_reset, 1, 1, 0, st, ws->kt, ws->kt_ws, ws->kt_we);
   HIP_TRY(hipEventRecord(ws->ev0, st));
   const bool plain = mode == BCMI_MODE_PLAIN;
   const int W = std::max(1, std::min(64, prm.width));

@@ -162,7 +162,9 @@ int bcmi_get_bc(bcmi_ctx *ctx, double *bc);
  * degrees, DAG arcs, levels (arrays of length >= st->sources_run). */
 int bcmi_get_sweep_stats(bcmi_ctx *ctx, int64_t *n_s, int64_t *m_s, int64_t *a_s, int32_t *levels);
 
-/* Duration (ms, HIP events around the kernel inside the captured level loop)
+/* Duration (ms; per launch, first wave entry to last wave exit on the
+ * device's constant-rate clock, summed on the device inside the captured
+ * level loop)
  * and launch count of the kernel selected by parameter "ktime" during the
  * last run (1 batched sigma, 2 batched backward, 3/4 batched top-down/
  * bottom-up, 5/6/7 plain top-down/bottom-up/backward).  No reference
