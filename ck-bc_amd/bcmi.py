@@ -24,7 +24,8 @@ SRC_EXACT, SRC_EXPAND = 0, 1
 
 EXPORTS = [
     "bcmi_last_error", "bcmi_version",
-    "bcmi_graph_read_edgelist", "bcmi_graph_from_edges", "bcmi_gen_rmat", "bcmi_gen_pendant_trees",
+    "bcmi_graph_read_edgelist", "bcmi_graph_save", "bcmi_graph_load",
+    "bcmi_graph_from_edges", "bcmi_gen_rmat", "bcmi_gen_pendant_trees",
     "bcmi_gen_block_chain", "bcmi_sample_sources", "bcmi_graph_free",
     "bcmi_create", "bcmi_destroy", "bcmi_set_param", "bcmi_get_param", "bcmi_upload_graph",
     "bcmi_get_peel", "bcmi_get_labels", "bcmi_bc_reset", "bcmi_bc_run", "bcmi_get_bc",
@@ -80,6 +81,8 @@ def lib():
         L.bcmi_graph_read_edgelist.argtypes = [ctypes.c_char_p, u64, ctypes.c_int, gpp, ctypes.POINTER(u64),
                                                ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.bcmi_graph_from_edges.argtypes = [vp, vp, i64, i64, ctypes.c_int, gpp]
+        L.bcmi_graph_save.argtypes = [ctypes.POINTER(_Graph), ctypes.c_char_p, u64, i64]
+        L.bcmi_graph_load.argtypes = [ctypes.c_char_p, gpp, ctypes.POINTER(u64), ctypes.POINTER(i64)]
         L.bcmi_gen_rmat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     u64, ctypes.c_int, gpp]
         L.bcmi_gen_pendant_trees.argtypes = [ctypes.POINTER(_Graph), ctypes.c_double, u64, gpp]
@@ -161,6 +164,18 @@ class Graph:
         g = Graph._take(gp)
         g.header = (hn.value, he.value) if hn.value else None
         return g
+
+    @staticmethod
+    def load(path):
+        """Reads a binary CSR cache; returns (graph, (source size, source mtime ns))."""
+        gp = ctypes.POINTER(_Graph)()
+        sz, mt = ctypes.c_uint64(0), ctypes.c_int64(0)
+        _check(lib().bcmi_graph_load(os.fsencode(path), ctypes.byref(gp), ctypes.byref(sz), ctypes.byref(mt)))
+        return Graph._take(gp), (sz.value, mt.value)
+
+    def save(self, path, src_size=0, src_mtime_ns=0):
+        v = self._view()
+        _check(lib().bcmi_graph_save(ctypes.byref(v), os.fsencode(path), src_size, src_mtime_ns))
 
     @staticmethod
     def from_edges(edges, n, directed=False):

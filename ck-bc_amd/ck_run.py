@@ -5,6 +5,12 @@ the program's tmp/ directory, executes the command there and honours
 ignore_return_code.
 
     python ck_run.py program/bc-hip --dataset /path/to/facebook_combined.txt
+    python ck_run.py program/bc-hip --cmd rmat22 --dataset-entry dataset/graph-rmat-s22
+
+A dataset entry (a directory with .cm/meta.json) must carry every tag the
+run command asks for (`dataset_tags`), as CK's selection does; its first
+`dataset_files` entry is the graph, created with the entry's `generate`
+command when it is missing.
 """
 import argparse
 import json
@@ -23,7 +29,8 @@ def expand(cmd, env):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("program", help="program entry directory (holds .cm/meta.json)")
-    ap.add_argument("--dataset", required=True, help="graph file selected for the run")
+    ap.add_argument("--dataset", help="graph file selected for the run")
+    ap.add_argument("--dataset-entry", help="CK dataset entry directory (generated graphs)")
     ap.add_argument("--cmd", default="default")
     ap.add_argument("--extra", default="", help="extra command-line arguments")
     a = ap.parse_args(argv)
@@ -36,7 +43,23 @@ def main(argv=None):
         cc = expand(meta["compile_cmds"]["default"]["cmd"], {"script_ext": ".sh"})
         subprocess.check_call(shlex.split(cc), cwd=tmp, env=env)
     rc_meta = meta["run_cmds"][a.cmd]
-    d = os.path.abspath(a.dataset)
+    if a.dataset_entry:
+        ent = os.path.abspath(a.dataset_entry)
+        dmeta = json.load(open(os.path.join(ent, ".cm", "meta.json")))
+        missing = set(rc_meta.get("dataset_tags", [])) - set(dmeta.get("tags", []))
+        if missing:
+            print(f"ck_run: dataset entry lacks tags {sorted(missing)}", file=sys.stderr)
+            return 2
+        d = os.path.join(ent, dmeta["dataset_files"][0])
+        if not os.path.exists(d):
+            gen = expand(dmeta["generate"]["cmd"], {"ck_bc_amd": os.path.dirname(os.path.abspath(__file__)),
+                                                    "dataset_file": d})
+            print("ck_run: generating", gen, flush=True)
+            subprocess.check_call(shlex.split(gen), cwd=ent)
+    elif a.dataset:
+        d = os.path.abspath(a.dataset)
+    else:
+        ap.error("--dataset or --dataset-entry is required")
     cmd = expand(rc_meta["run_time"]["run_cmd_main"], {
         "BIN_FILE": "./" + meta["target_file"],
         "dataset_path": os.path.dirname(d) + os.sep,

@@ -13,6 +13,7 @@
 // -S/-E generate an R-MAT graph (disabled in the reference, bc2d.c:664).
 // Errors print a message and exit 1 like prexit (bc2d.c:64-72).
 #include <getopt.h>
+#include <sys/stat.h>
 
 #include <chrono>
 #include <cinttypes>
@@ -48,6 +49,8 @@ static void usage(const char* p) {
       "\t\t -F <file>          read the sources from a file\n"
       "\t\t -P                 full-precision result file\n"
       "\t\t -X                 with -H 1: exact BC over exactly the listed sources\n"
+      "\t\t -C <file>          binary CSR cache of the -f graph: used when it matches\n"
+      "\t\t                    the text file's size and time, (re)written otherwise\n"
       "\n",
       p);
 }
@@ -67,12 +70,12 @@ int main(int argc, char** argv) {
   int R = 0, C = 0, heuristic = 0, debug = 0, undirected = 1, gread = -1, scale = 21, edgef = 16;
   uint64_t N = 0, startv = 0, nbfs = 1, seed = 0;
   int have_seed = 0, full_prec = 0, exact = 0, mode = BCMI_MODE_BATCHED;
-  std::string gfile, ofile, sfile;
+  std::string gfile, ofile, sfile, cfile;
   printf("MPI Tasks 1\n");
   for (int i = 0; i < argc; i++) printf(" %s", argv[i]);
   printf("\n\n");
   int opt;
-  while ((opt = getopt(argc, argv, "o:p:amhDR:Uf:n:r:S:E:N:H:B:s:F:PX")) != -1) {
+  while ((opt = getopt(argc, argv, "o:p:amhDR:Uf:n:r:S:E:N:H:B:s:F:PXC:")) != -1) {
     switch (opt) {
       case 'H':
         if (sscanf(optarg, "%d", &heuristic) != 1) die("Invalid Heuristic Option (-H): %s\n", optarg);
@@ -128,6 +131,7 @@ int main(int argc, char** argv) {
       case 'F': sfile = optarg; break;
       case 'P': full_prec = 1; break;
       case 'X': exact = 1; break;
+      case 'C': cfile = optarg; break;
       default: usage(argv[0]);
     }
   }
@@ -158,7 +162,47 @@ int main(int argc, char** argv) {
 
   double t0 = now_s();
   bcmi_graph* g = nullptr;
-  if (gread) {
+  // binary CSR cache: valid when it records the text file's size and mtime
+  // and matches -n and -U
+  uint64_t src_size = 0;
+  int64_t src_mtime = 0;
+  if (gread && !cfile.empty()) {
+    struct stat stt;
+    if (stat(gfile.c_str(), &stt) == 0) {
+      src_size = (uint64_t)stt.st_size;
+      src_mtime = (int64_t)stt.st_mtim.tv_sec * 1000000000ll + stt.st_mtim.tv_nsec;
+      uint64_t cs = 0;
+      int64_t cm = 0;
+      bcmi_graph* cg = nullptr;
+      if (bcmi_graph_load(cfile.c_str(), &cg, &cs, &cm) == BCMI_OK) {
+        if (cs == src_size && cm == src_mtime && (uint64_t)cg->n == N && cg->directed == (undirected ? 0 : 1)) {
+          g = cg;
+          printf("Graph read from CSR cache %s\n", cfile.c_str());
+        } else {
+          bcmi_graph_free(cg);
+        }
+      }
+    }
+  }
+  bool csr_input = false;
+  if (gread && !g) {
+    // -f may name a binary CSR file (as written by -C or gen_dataset.py)
+    char mg[8] = {0};
+    FILE* fp = fopen(gfile.c_str(), "rb");
+    if (fp) {
+      csr_input = fread(mg, 1, 8, fp) == 8 && memcmp(mg, "BCMICSR1", 8) == 0;
+      fclose(fp);
+    }
+    if (csr_input) {
+      CK(bcmi_graph_load(gfile.c_str(), &g, nullptr, nullptr));
+      if ((uint64_t)g->n != N) die("CSR file %s holds %" PRId64 " vertices, -n says %" PRIu64 "\n", gfile.c_str(), g->n, N);
+      if (g->directed != (undirected ? 0 : 1)) die("CSR file %s: directedness differs from -U\n", gfile.c_str());
+      printf("Graph read from CSR file %s\n", gfile.c_str());
+    }
+  }
+  if (g) {
+    // loaded from the cache or a CSR file
+  } else if (gread) {
     uint64_t bad = 0, hn = 0, he = 0;
     int rc = bcmi_graph_read_edgelist(gfile.c_str(), N, undirected ? 0 : 1, &g, &bad, &hn, &he);
     if (rc != BCMI_OK) {
@@ -168,6 +212,7 @@ int main(int argc, char** argv) {
       die("%s\n", bcmi_last_error());
     }
     if (hn) printf("N=%" PRIu64 " E=%" PRIu64 "\n", hn, he);
+    if (!cfile.empty() && src_size) CK(bcmi_graph_save(g, cfile.c_str(), src_size, src_mtime));
   } else {
     CK(bcmi_gen_rmat(scale, edgef, 0.57, 0.19, 0.19, seed ? seed : 1, 1, &g));
   }

@@ -13,6 +13,8 @@ namespace bcmi {
 int set_error(int code, const std::string& msg);
 
 bcmi_graph* graph_alloc(int64_t n, int64_t m, int directed);
+int graph_save(const bcmi_graph* g, const char* path, uint64_t src_size, int64_t src_mtime_ns);
+int graph_load(const char* path, bcmi_graph** out, uint64_t* src_size, int64_t* src_mtime_ns);
 bcmi_graph* csr_from_arcs(int64_t n, const int32_t* tail, const int32_t* head, int64_t k, int mirror,
                           int directed);
 void relabel_by_degree(int64_t n, const int64_t* off, const int32_t* adj, std::vector<int32_t>& n2o,

@@ -1499,12 +1499,27 @@ __global__ void k_cc_check(const int64_t* off, const int32_t* adj, const int32_t
   }
   if (__ballot(any) && lane_id() == 0) atomicOr(bad, 1);
 }
+// Vertices and traversed arcs per component.  Most vertices share one label
+// (the giant component): a wave whose lanes all carry lane 0's label adds
+// its sums with one atomic, other waves per lane.
 __global__ void k_cc_sizes(const int32_t* label, int64_t n, const int64_t* coff, int64_t* csize, int64_t* carcs) {
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-    int32_t l = label[v];
-    atomicAdd((u64*)&csize[l], 1ull);
-    int64_t dg = coff[v + 1] - coff[v];
-    if (dg) atomicAdd((u64*)&carcs[l], (u64)dg);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const int64_t v = base + threadIdx.x;
+    const bool ok = v < n;
+    const int32_t l = ok ? label[v] : -1;
+    const int64_t dg = ok ? coff[v + 1] - coff[v] : 0;
+    const int32_t l0 = rl32(l, 0);
+    if (__all(!ok || l == l0)) {
+      long long c = wave_sum_ll(ok ? 1 : 0), a = wave_sum_ll(dg);
+      if (lane_id() == 0 && l0 >= 0) {
+        atomicAdd((u64*)&csize[l0], (u64)c);
+        if (a) atomicAdd((u64*)&carcs[l0], (u64)a);
+      }
+    } else if (ok) {
+      atomicAdd((u64*)&csize[l], 1ull);
+      if (dg) atomicAdd((u64*)&carcs[l], (u64)dg);
+    }
   }
 }
 

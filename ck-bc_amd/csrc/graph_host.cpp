@@ -386,6 +386,127 @@ int gen_block_chain(int64_t nblocks, int block, double avgdeg, uint64_t seed, bc
   return BCMI_OK;
 }
 
+// ------------------------------------------------------------ CSR cache
+// Binary CSR file: a 64-byte header (magic, version, directed, n, m, size
+// and mtime of the text file it was built from, checksum), then off[n+1]
+// as int64 and adj[m] as int32, little-endian.  Loading checks the header,
+// the file size, the checksum and the CSR invariants.
+static const char kCsrMagic[8] = {'B', 'C', 'M', 'I', 'C', 'S', 'R', '1'};
+struct CsrHeader {
+  char magic[8];
+  uint32_t version, directed;
+  int64_t n, m;
+  uint64_t src_size;
+  int64_t src_mtime_ns;
+  uint64_t checksum;
+  uint64_t reserved;
+};
+static_assert(sizeof(CsrHeader) == 64, "header layout");
+
+// Order-dependent hash of a byte range: FNV-1a per 1 MiB chunk, chunks mixed
+// with their index (parallel over chunks).
+static uint64_t range_hash(const unsigned char* p, size_t len) {
+  const size_t C = 1 << 20;
+  const int64_t nc = (int64_t)((len + C - 1) / C);
+  uint64_t acc = 0;
+#pragma omp parallel for reduction(+ : acc) schedule(static)
+  for (int64_t c = 0; c < nc; c++) {
+    uint64_t h = 1469598103934665603ull;
+    const size_t lo = (size_t)c * C, hi = std::min(len, lo + C);
+    for (size_t i = lo; i < hi; i++) h = (h ^ p[i]) * 1099511628211ull;
+    acc += splitmix64(h ^ (uint64_t)c);
+  }
+  return acc;
+}
+
+static uint64_t csr_checksum(const bcmi_graph* g) {
+  return splitmix64(range_hash((const unsigned char*)g->off, sizeof(int64_t) * (g->n + 1)) ^
+                    (range_hash((const unsigned char*)g->adj, sizeof(int32_t) * g->m) * 31));
+}
+
+int graph_save(const bcmi_graph* g, const char* path, uint64_t src_size, int64_t src_mtime_ns) {
+  CsrHeader h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, kCsrMagic, 8);
+  h.version = 1;
+  h.directed = (uint32_t)g->directed;
+  h.n = g->n;
+  h.m = g->m;
+  h.src_size = src_size;
+  h.src_mtime_ns = src_mtime_ns;
+  h.checksum = csr_checksum(g);
+  std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return set_error(BCMI_ERR_IO, std::string("cannot write ") + tmp + ": " + strerror(errno));
+  bool ok = fwrite(&h, sizeof h, 1, f) == 1 &&
+            fwrite(g->off, sizeof(int64_t), (size_t)g->n + 1, f) == (size_t)g->n + 1 &&
+            (g->m == 0 || fwrite(g->adj, sizeof(int32_t), (size_t)g->m, f) == (size_t)g->m);
+  ok = (fclose(f) == 0) && ok;
+  if (!ok || rename(tmp.c_str(), path) != 0) {
+    unlink(tmp.c_str());
+    return set_error(BCMI_ERR_IO, std::string("writing ") + path + " failed");
+  }
+  return BCMI_OK;
+}
+
+int graph_load(const char* path, bcmi_graph** out, uint64_t* src_size, int64_t* src_mtime_ns) {
+  int fd = open(path, O_RDONLY);
+  if (fd < 0) return set_error(BCMI_ERR_IO, std::string("cannot open ") + path + ": " + strerror(errno));
+  struct stat st;
+  if (fstat(fd, &st) != 0 || (size_t)st.st_size < sizeof(CsrHeader)) {
+    close(fd);
+    return set_error(BCMI_ERR_IO, std::string(path) + ": not a CSR cache (too short)");
+  }
+  CsrHeader h;
+  if (pread(fd, &h, sizeof h, 0) != (ssize_t)sizeof h || memcmp(h.magic, kCsrMagic, 8) != 0 || h.version != 1) {
+    close(fd);
+    return set_error(BCMI_ERR_IO, std::string(path) + ": not a CSR cache (bad magic or version)");
+  }
+  const uint64_t want = sizeof h + sizeof(int64_t) * (uint64_t)(h.n + 1) + sizeof(int32_t) * (uint64_t)h.m;
+  if (h.n < 1 || h.n > INT32_MAX || h.m < 0 || (uint64_t)st.st_size != want) {
+    close(fd);
+    return set_error(BCMI_ERR_IO, std::string(path) + ": size does not match its header");
+  }
+  bcmi_graph* g = graph_alloc(h.n, h.m, (int)h.directed);
+  if (!g) {
+    close(fd);
+    return set_error(BCMI_ERR_NOMEM, "out of host memory");
+  }
+  auto rd = [&](void* dst, size_t bytes, off_t at) {
+    size_t got = 0;
+    while (got < bytes) {
+      ssize_t r = pread(fd, (char*)dst + got, bytes - got, at + (off_t)got);
+      if (r <= 0) return false;
+      got += (size_t)r;
+    }
+    return true;
+  };
+  bool ok = rd(g->off, sizeof(int64_t) * (size_t)(h.n + 1), sizeof h) &&
+            rd(g->adj, sizeof(int32_t) * (size_t)h.m, (off_t)(sizeof h + sizeof(int64_t) * (size_t)(h.n + 1)));
+  close(fd);
+  if (!ok) {
+    bcmi_graph_free(g);
+    return set_error(BCMI_ERR_IO, std::string(path) + ": read failed");
+  }
+  if (csr_checksum(g) != h.checksum) {
+    bcmi_graph_free(g);
+    return set_error(BCMI_ERR_IO, std::string(path) + ": checksum mismatch");
+  }
+  bool valid = g->off[0] == 0 && g->off[h.n] == h.m;
+  for (int64_t v = 0; valid && v < h.n; v++) valid = g->off[v] <= g->off[v + 1];
+  int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad)
+  for (int64_t e = 0; e < h.m; e++) bad += (g->adj[e] < 0 || g->adj[e] >= h.n);
+  if (!valid || bad) {
+    bcmi_graph_free(g);
+    return set_error(BCMI_ERR_IO, std::string(path) + ": invalid CSR");
+  }
+  if (src_size) *src_size = h.src_size;
+  if (src_mtime_ns) *src_mtime_ns = h.src_mtime_ns;
+  *out = g;
+  return BCMI_OK;
+}
+
 // Relabels vertices by descending degree (ties by id): new id i holds old
 // vertex n2o[i].  The adjacency of each vertex keeps its original order
 // (entries renamed, not re-sorted), so per-vertex sums keep their order.
@@ -436,6 +557,16 @@ int bcmi_graph_read_edgelist(const char* path, uint64_t n, int directed, bcmi_gr
                              uint64_t* bad_line, uint64_t* hdr_nodes, uint64_t* hdr_edges) {
   if (!path || !out) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
   return bcmi::read_edgelist(path, n, directed, out, bad_line, hdr_nodes, hdr_edges);
+}
+
+int bcmi_graph_save(const bcmi_graph* g, const char* path, uint64_t src_size, int64_t src_mtime_ns) {
+  if (!g || !path) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
+  return bcmi::graph_save(g, path, src_size, src_mtime_ns);
+}
+
+int bcmi_graph_load(const char* path, bcmi_graph** out, uint64_t* src_size, int64_t* src_mtime_ns) {
+  if (!path || !out) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
+  return bcmi::graph_load(path, out, src_size, src_mtime_ns);
 }
 
 int bcmi_graph_from_edges(const int32_t* tails, const int32_t* heads, int64_t k, int64_t n, int directed,

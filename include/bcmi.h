@@ -90,6 +90,14 @@ int bcmi_version(void);
  * "# Nodes: N Edges: E" header values are returned when present (c:220-226). */
 int bcmi_graph_read_edgelist(const char *path, uint64_t n, int directed, bcmi_graph **out,
                              uint64_t *bad_line, uint64_t *hdr_nodes, uint64_t *hdr_edges);
+/* Binary CSR cache (no counterpart in the reference, which re-parses the
+ * text with read_graph/norm_graph/build_csc on every run, c:175-254,
+ * c:430-453, c:472-501).  save writes header + int64 offsets + int32
+ * adjacency with a checksum; src_size/src_mtime_ns record the text file the
+ * CSR came from (0 if none) so a caller can tell a stale cache.  load checks
+ * magic, size, checksum and CSR invariants (BCMI_ERR_IO otherwise). */
+int bcmi_graph_save(const bcmi_graph *g, const char *path, uint64_t src_size, int64_t src_mtime_ns);
+int bcmi_graph_load(const char *path, bcmi_graph **out, uint64_t *src_size, int64_t *src_mtime_ns);
 /* Same pipeline from in-memory arcs tails[i] -> heads[i]. */
 int bcmi_graph_from_edges(const int32_t *tails, const int32_t *heads, int64_t k, int64_t n,
                           int directed, bcmi_graph **out);

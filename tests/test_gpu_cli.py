@@ -72,3 +72,47 @@ def test_ck_runner(tmp_path):
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "BC rounds: 1000" in r.stdout
+
+
+def test_csr_cache(tmp_path):
+    """-C: the first run parses the text and writes the binary CSR, the next
+    reads it (same result file); a changed text file is parsed again."""
+    fn, f = write_facebook(tmp_path)
+    cache = tmp_path / "fb.csr"
+    outs = []
+    for k in range(3):
+        if k == 2:
+            os.utime(fn, ns=(0, 10**18))                # the text changed (mtime)
+        out = tmp_path / f"res{k}"
+        r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "4039", "-N", "200", "-o", str(out),
+                            "-C", str(cache)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert ("Graph read from CSR cache" in r.stdout) == (k == 1)
+        outs.append(open(str(out) + "_1X1_0.log").read())
+    assert cache.exists()
+    assert outs[0] == outs[1] == outs[2]
+
+
+def test_ck_run_generated_dataset_entry(tmp_path):
+    """CK-style run of a generated dataset entry (R-MAT scale 10, all 1,024
+    sources): the runner creates the binary CSR from the entry's recipe and
+    the program reads it through -f; BC matches the oracle."""
+    import shutil
+    sys.path.insert(0, os.path.join(ROOT, "ck-bc_amd"))
+    import bcmi
+    from oracle import oracle as O
+    ent = tmp_path / "graph-rmat-s10"
+    shutil.copytree(os.path.join(ROOT, "ck-bc_amd", "dataset", "graph-rmat-s10"), ent)
+    out = tmp_path / "res"
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "ck-bc_amd", "ck_run.py"),
+                        os.path.join(ROOT, "ck-bc_amd", "program", "bc-hip"), "--cmd", "rmat10",
+                        "--dataset-entry", str(ent), "--extra", f"-o {out} -P"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Graph read from CSR file" in r.stdout
+    g = bcmi.Graph.rmat(10, 16, seed=20250101)
+    lines = open(str(out) + "_1X1_0.log").read().splitlines()
+    got = np.array([float(l.split("\t")[1]) for l in lines[3:]])
+    ref = O.brandes(g.off, g.adj, np.arange(g.n)) / 2.0
+    nz = ref != 0
+    assert np.all(got[~nz] == 0) and np.max(np.abs(got[nz] - ref[nz]) / ref[nz]) < 1e-9

@@ -142,3 +142,54 @@ def test_sample_sources():
     assert np.all(g.degrees()[s] > 0)
     assert np.array_equal(s, g.sample_sources(500, seed=7))
     assert not np.array_equal(s, g.sample_sources(500, seed=8))
+
+
+def test_csr_cache_roundtrip_and_checks(tmp_path):
+    for g in (bcmi.Graph.rmat(10, 16, seed=3), bcmi.Graph.from_edges(*G.random_graph(300, 900, 2), directed=True)):
+        p = tmp_path / "g.csr"
+        g.save(str(p), 1234, 5678)
+        h, prov = bcmi.Graph.load(str(p))
+        assert prov == (1234, 5678)
+        assert h.n == g.n and h.directed == g.directed
+        assert np.array_equal(h.off, g.off) and np.array_equal(h.adj, g.adj)
+    raw = bytearray(open(p, "rb").read())
+    bad = tmp_path / "bad.csr"
+    for mutate in (lambda b: b[:len(b) - 4],                                   # truncated
+                   lambda b: b[:70] + bytes([b[70] ^ 1]) + b[71:],             # flipped bit
+                   lambda b: b"XXXXXXXX" + b[8:]):                             # bad magic
+        open(bad, "wb").write(bytes(mutate(raw)))
+        with pytest.raises(bcmi.BcmiError) as ex:
+            bcmi.Graph.load(str(bad))
+        assert ex.value.code == bcmi.ERR_IO
+    with pytest.raises(bcmi.BcmiError) as ex:
+        bcmi.Graph.load(str(tmp_path / "missing.csr"))
+    assert ex.value.code == bcmi.ERR_IO
+
+
+def test_ck_dataset_entries_and_generator(tmp_path):
+    """Every run command of the CK program finds a dataset entry carrying its
+    tags; the generator writes the graph the benchmarks use."""
+    import json
+    import subprocess
+    import sys
+    pkg = os.path.join(ROOT, "ck-bc_amd")
+    prog = json.load(open(os.path.join(pkg, "program", "bc-hip", ".cm", "meta.json")))
+    entries = {}
+    for d in os.listdir(os.path.join(pkg, "dataset")):
+        entries[d] = json.load(open(os.path.join(pkg, "dataset", d, ".cm", "meta.json")))
+    for name, rc in prog["run_cmds"].items():
+        if name == "default":
+            continue                       # the reference's facebook dataset entry
+        tags = set(rc["dataset_tags"])
+        assert any(tags <= set(m["tags"]) for m in entries.values()), name
+    out = tmp_path / "r10.csr"
+    subprocess.check_call([sys.executable, os.path.join(pkg, "gen_dataset.py"), "rmat", "--scale", "10",
+                           "--edgefactor", "16", "--seed", "20250101", "--out", str(out)])
+    h, _ = bcmi.Graph.load(str(out))
+    g = bcmi.Graph.rmat(10, 16, seed=20250101)
+    assert np.array_equal(h.off, g.off) and np.array_equal(h.adj, g.adj)
+    txt = tmp_path / "r10.txt"
+    subprocess.check_call([sys.executable, os.path.join(pkg, "gen_dataset.py"), "rmat", "--scale", "10",
+                           "--edgefactor", "16", "--seed", "20250101", "--out", str(txt), "--text"])
+    t = bcmi.Graph.read_edgelist(str(txt), g.n)
+    assert np.array_equal(t.off, g.off) and np.array_equal(t.adj, g.adj)
