@@ -27,6 +27,8 @@ struct Params {
   int chunk = 256;          // max edges per work item
   int64_t alpha_plain = 1000;   // x1000; bottom-up when frontier arcs > alpha * unvisited arcs
   int64_t alpha_batched = 1000; // x1000
+  int64_t beta_plain = 16;      // bottom-up only when frontier arcs >= all arcs / beta
+  int64_t beta_batched = 16;
   int steps = 16;           // levels per captured graph
   int sgrid = 512;          // blocks per kernel in the small-grid (sparse-level) graphs
   int64_t sparse_verts = 16384; // levels below this many vertices replay the small-grid graphs

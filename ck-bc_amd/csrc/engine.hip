@@ -198,6 +198,7 @@ struct GA {
   const uint64_t* w;
   int chunk;
   int64_t n;
+  int64_t m;  // traversed arcs
   int flags;  // KF_*
 };
 
@@ -220,6 +221,7 @@ struct PA {
   const double* multbuf;
   double* bc;
   int64_t alpha;
+  int64_t beta;   // bottom-up only when the frontier holds >= 1/beta of all arcs
 };
 
 struct BA {
@@ -245,6 +247,7 @@ struct BA {
   u64* st_a;
   double* dbg_delta;
   int64_t alpha;
+  int64_t beta;   // bottom-up only when the frontier holds >= 1/beta of all arcs
   int64_t n;
   double* lscale;      // [level][64] scale factor m of each level (scaled path counts)
   int64_t lscale_cap;  // levels it can hold
@@ -477,7 +480,9 @@ __global__ void k_plain_decide(GA g, PA p, int allow_bu) {
   // unvisited vertex of the component (no early exit: BC needs all parents).
   c->arcs_seen += c->arcs_cur;
   int64_t unvisited = c->arcs_reach - c->arcs_seen;
-  bool bu = allow_bu && !g.directed && (double)c->arcs_cur * 1000.0 > (double)p.alpha * (double)unvisited;
+  // bottom-up scans every vertex's records, so a small frontier stays top-down
+  bool bu = allow_bu && !g.directed && (double)c->arcs_cur * 1000.0 > (double)p.alpha * (double)unvisited &&
+            (double)c->arcs_cur * (double)p.beta >= (double)g.m;
   c->mode = bu ? 1 : 0;
   if (bu) c->n_bu++; else c->n_td++;
 }
@@ -801,7 +806,10 @@ __global__ void k_ms_decide(GA g, BA b, int allow_bu) {
     }
   }
   int64_t incomplete = c->arcs_reach - c->arcs_seen;
-  bool bu = allow_bu && !g.directed && (double)c->arcs_cur * 1000.0 > (double)b.alpha * (double)incomplete;
+  // bottom-up scans the records of every incomplete vertex (and tests all
+  // others), so a small frontier stays top-down
+  bool bu = allow_bu && !g.directed && (double)c->arcs_cur * 1000.0 > (double)b.alpha * (double)incomplete &&
+            (double)c->arcs_cur * (double)b.beta >= (double)g.m;
   c->mode = bu ? 1 : 0;
   if (bu) c->n_bu++; else c->n_td++;
 }
@@ -2036,7 +2044,7 @@ static GA make_ga(const DevGraph& g, const Params& prm) {
   a.label = g.label; a.comp_arcs = g.comp_arcs;
   a.all_v = g.all_v; a.all_j = g.all_j; a.nall = g.nall;
   a.w = g.heuristic ? (const uint64_t*)g.w : nullptr;
-  a.chunk = g.chunk; a.n = g.n;
+  a.chunk = g.chunk; a.n = g.n; a.m = g.m;
   a.flags = prm.kflags;
   return a;
 }
@@ -2044,7 +2052,7 @@ static PA make_pa(Workspace* ws, const Params& prm) {
   PA p;
   p.ctl = ws->ctl; p.depth = ws->depth; p.sigma = ws->sigma1; p.delta = ws->delta1; p.coef = ws->coef1;
   p.done = ws->done; p.rec_v = ws->rec_v; p.rec_j = ws->rec_j; p.cap = ws->cap; p.rec_lo = ws->rec_lo;
-  p.srcbuf = ws->srcbuf; p.multbuf = ws->multbuf; p.bc = ws->bc; p.alpha = prm.alpha_plain;
+  p.srcbuf = ws->srcbuf; p.multbuf = ws->multbuf; p.bc = ws->bc; p.alpha = prm.alpha_plain; p.beta = prm.beta_plain;
   return p;
 }
 static BA make_ba(Workspace* ws, const Params& prm) {
@@ -2053,7 +2061,7 @@ static BA make_ba(Workspace* ws, const Params& prm) {
   b.rec_v = ws->rec_v; b.rec_j = ws->rec_j; b.rec_mask = ws->rec_mask; b.cap = ws->cap; b.rec_lo = ws->rec_lo;
   b.compmask = ws->compmask; b.srcbuf = ws->srcbuf; b.multbuf = ws->multbuf; b.bc = ws->bc;
   b.st_n = ws->stats; b.st_m = ws->stats + 64; b.st_a = ws->stats + 128; b.dbg_delta = ws->dbg_delta;
-  b.alpha = prm.alpha_batched; b.n = ws->n;
+  b.alpha = prm.alpha_batched; b.beta = prm.beta_batched; b.n = ws->n;
   b.lscale = ws->lscale; b.lscale_cap = ws->lscale_cap; b.lvmax = ws->lvmax;
   return b;
 }
@@ -2325,7 +2333,8 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
 
 static bool same_launch(const Params& a, const Params& b) {
   return a.block == b.block && a.grid == b.grid && a.chunk == b.chunk && a.alpha_plain == b.alpha_plain &&
-         a.alpha_batched == b.alpha_batched && a.steps == b.steps && a.ktime == b.ktime &&
+         a.alpha_batched == b.alpha_batched && a.beta_plain == b.beta_plain && a.beta_batched == b.beta_batched &&
+         a.steps == b.steps && a.ktime == b.ktime &&
          a.kflags == b.kflags && a.sgrid == b.sgrid;
 }
 
