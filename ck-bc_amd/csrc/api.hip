@@ -112,6 +112,9 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "alpha_batched") {
     if (v < 0) return set_error(BCMI_ERR_ARG, "alpha must be >= 0");
     p.alpha_batched = v;
+  } else if (k == "gamma_push") {
+    if (v < 0) return set_error(BCMI_ERR_ARG, "gamma_push must be >= 0");
+    p.gamma_push = v;
   } else if (k == "beta_plain" || k == "beta_batched") {
     if (v < 1) return set_error(BCMI_ERR_ARG, "beta must be >= 1");
     (k == "beta_plain" ? p.beta_plain : p.beta_batched) = v;
@@ -177,6 +180,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "alpha_batched") *v = p.alpha_batched;
   else if (k == "beta_plain") *v = p.beta_plain;
   else if (k == "beta_batched") *v = p.beta_batched;
+  else if (k == "gamma_push") *v = p.gamma_push;
   else if (k == "steps") *v = p.steps;
   else if (k == "width") *v = p.width;
   else if (k == "ktime") *v = p.ktime;

@@ -29,6 +29,8 @@ struct Params {
   int64_t alpha_batched = 1000; // x1000
   int64_t beta_plain = 16;      // bottom-up only when frontier arcs >= all arcs / beta
   int64_t beta_batched = 16;
+  int64_t gamma_push = 8;       // batched top-down levels with frontier arcs < all arcs / gamma push
+                                // path counts from the frontier (0: always pull from parents)
   int steps = 16;           // levels per captured graph
   int sgrid = 512;          // blocks per kernel in the small-grid (sparse-level) graphs
   int64_t sparse_verts = 16384; // levels below this many vertices replay the small-grid graphs

@@ -27,7 +27,7 @@ struct Ctl {
   int32_t src_label;  // component of the plain-sweep source
   int32_t width;      // sources in the batch
   int32_t scaled;     // batched: path counts held as FP64 scaled per level
-  int32_t pad_;
+  int32_t push;       // batched: this level's path counts are pushed from the frontier (k_ms_push)
   int64_t rec_count;  // records appended so far
   int64_t arcs_next;  // degree sum of vertices appended during this step
   int64_t verts_next;

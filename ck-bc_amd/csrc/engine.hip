@@ -248,6 +248,7 @@ struct BA {
   double* dbg_delta;
   int64_t alpha;
   int64_t beta;   // bottom-up only when the frontier holds >= 1/beta of all arcs
+  int64_t gamma;  // top-down levels whose frontier holds < 1/gamma of all arcs push path counts
   int64_t n;
   double* lscale;      // [level][64] scale factor m of each level (scaled path counts)
   int64_t lscale_cap;  // levels it can hold
@@ -724,7 +725,7 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled) {
     c->arcs_next = 0; c->verts_next = 0; c->arcs_cur = 0; c->verts_cur = 0;
     c->arcs_seen = 0; c->sum_a = 0; c->n_td = 0; c->n_bu = 0;
     c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 0; c->uni_arcs = 0;
-    c->sigma_ovf = 0; c->width = k; c->scaled = scaled;
+    c->sigma_ovf = 0; c->width = k; c->scaled = scaled; c->push = 0;
     b.rec_lo[0] = 0;
   }
   b.st_n[i] = 0; b.st_m[i] = 0; b.st_a[i] = 0;
@@ -812,6 +813,9 @@ __global__ void k_ms_decide(GA g, BA b, int allow_bu) {
             (double)c->arcs_cur * (double)b.beta >= (double)g.m;
   c->mode = bu ? 1 : 0;
   if (bu) c->n_bu++; else c->n_td++;
+  // Small top-down frontiers push path counts along the arcs they expand
+  // (the next level's own arcs, many more, are then not scanned for parents).
+  c->push = !bu && b.gamma > 0 && (double)c->arcs_cur * (double)b.gamma < (double)g.m;
 }
 
 // Zeroes the sigma columns of the sources that have not reached v yet, for
@@ -867,7 +871,7 @@ __device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
         }
       }
       app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
-      zero_multi_sigma(won && dg > g.chunk, v, b);
+      zero_multi_sigma(won && dg > g.chunk && !c->push, v, b);
     }
   }
   app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
@@ -951,6 +955,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   int ovf = 0;
   long long done_arcs = 0, uv = 0, ua = 0;
   const bool scaled = c->scaled;
+  const bool push = c->push;  // the top-down kernel already summed the parents
   const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   double mx = 0.0;
   for (int64_t r = lo + gw; r < hi; r += tw) {
@@ -960,7 +965,9 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
     int64_t deg = g.off[v + 1] - g.off[v];
     int nch = nchunks(deg, g.chunk);
     int64_t e0, e1;
-    if (!g.directed) {
+    if (push) {
+      e0 = e1 = 0;  // no parent scan
+    } else if (!g.directed) {
       e0 = g.off[v] + (int64_t)j * g.chunk;
       e1 = min(g.off[v + 1], e0 + g.chunk);
     } else {  // parents come from the transposed lists, split evenly over v's records
@@ -1014,7 +1021,9 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
       }
     }
     u64* row = b.sigma + (int64_t)v * 64 + lane;
-    if (scaled) {
+    if (push) {
+      if (mine && j == 0) *row = 0ull;  // k_ms_push adds the parents' counts next
+    } else if (scaled) {
       double raw = accf * inv_m;
       if (mine) {
         if (nch == 1) *row = (u64)__double_as_longlong(raw);
@@ -1068,6 +1077,70 @@ __global__ void __launch_bounds__(256) k_ms_sigma(GA g, BA b, KT* kt) {
   k_ms_sigma_body(g, b);
   kt_exit(kt);
 }
+
+// Pushed path counts (levels the decide kernel marked): for every arc u->v
+// out of a level-d frontier record, sigma[v][i] += sigma[u][i] for the
+// sources i with u on level d and v on level d+1 (rows of level d+1 were
+// zeroed by k_ms_sigma).  The frontier's arcs are far fewer than the next
+// level's, which the pull in k_ms_sigma would scan.  Exact counts use
+// returning atomics to detect a wrap; scaled counts add raw values and keep
+// the level maximum (every returned sum is a lower bound of a final one).
+__device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (c->fdone || !c->push) return;
+  const int d = c->level;
+  const u64* nxt = b.lm + (int64_t)((d + 1) % 3) * b.n;
+  const int64_t lo = b.rec_lo[d], hi = b.rec_lo[d + 1];
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  const bool scaled = c->scaled;
+  const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
+  u64 ca = 0;
+  int ovf = 0;
+  double mx = 0.0;
+  for (int64_t r = lo + gw; r < hi; r += tw) {
+    const int32_t u = b.rec_v[r];
+    const int j = b.rec_j[r];
+    const u64 F = b.rec_mask[r - j];
+    const int64_t e0 = g.off[u] + (int64_t)j * g.chunk;
+    const int64_t e1 = min(g.off[u + 1], e0 + g.chunk);
+    const u64 sb = ((F >> lane) & 1) ? b.sigma[(int64_t)u * 64 + lane] : 0ull;
+    const double sf = scaled ? __longlong_as_double((long long)sb) * inv_m : 0.0;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      const int64_t e = eb + lane;
+      int32_t v = 0;
+      u64 bits = 0;
+      if (e < e1) {
+        v = adj_at(g, g.adj, e);
+        bits = F & nxt[v];
+      }
+      u64 P = __ballot(bits != 0);
+      while (P) {
+        const int q = __ffsll((long long)P) - 1;
+        P &= P - 1;
+        const int32_t vq = rl32(v, q);
+        const u64 bq = rl64(bits, q);
+        if ((bq >> lane) & 1) {
+          u64* row = b.sigma + (int64_t)vq * 64 + lane;
+          ca++;
+          if (scaled) {
+            double o = atomicAdd((double*)row, sf);
+            mx = fmax(mx, o + sf);
+          } else {
+            u64 o = atomicAdd(row, sb);
+            if (o + sb < o) ovf = 1;
+          }
+        }
+      }
+    }
+  }
+  block_add_lanes(ca, b.st_a);
+  if (scaled) block_max_lanes((u64)__double_as_longlong(mx), b.lvmax);
+  if (ovf) c->sigma_ovf = 1;
+}
+
+__global__ void __launch_bounds__(256) k_ms_push(GA g, BA b) { k_ms_push_body(g, b); }
 
 // Backward level d for a batch: lane i accumulates, in adjacency order,
 // coef[z][i] * sigma[v][i] over the successors z of v for source i
@@ -2061,7 +2134,7 @@ static BA make_ba(Workspace* ws, const Params& prm) {
   b.rec_v = ws->rec_v; b.rec_j = ws->rec_j; b.rec_mask = ws->rec_mask; b.cap = ws->cap; b.rec_lo = ws->rec_lo;
   b.compmask = ws->compmask; b.srcbuf = ws->srcbuf; b.multbuf = ws->multbuf; b.bc = ws->bc;
   b.st_n = ws->stats; b.st_m = ws->stats + 64; b.st_a = ws->stats + 128; b.dbg_delta = ws->dbg_delta;
-  b.alpha = prm.alpha_batched; b.beta = prm.beta_batched; b.n = ws->n;
+  b.alpha = prm.alpha_batched; b.beta = prm.beta_batched; b.gamma = prm.gamma_push; b.n = ws->n;
   b.lscale = ws->lscale; b.lscale_cap = ws->lscale_cap; b.lvmax = ws->lvmax;
   return b;
 }
@@ -2309,6 +2382,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
           if (!sparse) ktacc(4, Gk);
           hipLaunchKernelGGL(k_ms_sigma, Gk, B, 0, st, ga, ba, tk(1));
           ktacc(1, Gk);
+          hipLaunchKernelGGL(k_ms_push, Gk, B, 0, st, ga, ba);
         } else {
           hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, ba.ctl);
           hipLaunchKernelGGL(k_ms_back, Gk, B, 0, st, ga, ba, tk(2));
@@ -2334,6 +2408,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
 static bool same_launch(const Params& a, const Params& b) {
   return a.block == b.block && a.grid == b.grid && a.chunk == b.chunk && a.alpha_plain == b.alpha_plain &&
          a.alpha_batched == b.alpha_batched && a.beta_plain == b.beta_plain && a.beta_batched == b.beta_batched &&
+         a.gamma_push == b.gamma_push &&
          a.steps == b.steps && a.ktime == b.ktime &&
          a.kflags == b.kflags && a.sgrid == b.sgrid;
 }

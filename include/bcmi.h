@@ -122,7 +122,10 @@ int bcmi_destroy(bcmi_ctx *ctx);
 /* Tunables: "block" (threads per block), "grid" (blocks), "chunk" (edges per
  * work item), "alpha_plain"/"alpha_batched" (x1000: top-down -> bottom-up
  * switch when frontier arcs > alpha * unvisited arcs), "beta_plain"/
- * "beta_batched" (and only when frontier arcs >= all arcs / beta), "steps" (levels per
+ * "beta_batched" (and only when frontier arcs >= all arcs / beta),
+ * "gamma_push" (batched top-down levels whose frontier holds fewer than
+ * all arcs / gamma push path counts from the frontier; 0 always pulls),
+ * "steps" (levels per
  * captured graph), "width" (sources per batch, 1..64), "ktime" (kernel timed
  * on the device, see bcmi_get_kernel_time), "sigma" (path counts: 0 exact
  * uint64, BCMI_ERR_OVERFLOW past 2^64-1; 1 FP64 scaled per level; 2 exact,
