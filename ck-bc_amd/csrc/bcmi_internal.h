@@ -38,7 +38,7 @@ struct Params {
   int strategy = 0;         // 0 auto, 1 device-wide level loop, 2 one workgroup per source
   int wg_depth = 128;       // auto: graphs whose hub BFS depth reaches this use a workgroup per source
   int64_t wg_mem = 16ll << 30;  // device bytes for the per-source state of the workgroup sweeps
-  int wg_slots = 512;       // at most this many sources in flight (workgroups)
+  int wg_slots = 2048;      // at most this many sources in flight (workgroups; 8 per CU fit)
   int width = 64;           // sources per batch
   int sigma = 2;            // path counts: 0 exact uint64 (error on overflow), 1 scaled FP64,
                             // 2 exact, batches that overflow are recomputed scaled

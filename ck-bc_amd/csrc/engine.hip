@@ -1413,28 +1413,38 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
         const u64 sv = wg_ld(sig + v);
         const int64_t e0 = g.off[v], e1 = g.off[v + 1];
         m_cnt += e1 - e0;
-        for (int64_t e = e0; e < e1; e++) {
-          const int32_t w = g.adj[e];
-          int32_t dw = wg_ld(depth + w);
-          if (dw < 0) {
-            int32_t expect = -1;
-            if (__hip_atomic_compare_exchange_strong(depth + w, &expect, d1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP)) {
-              const int pos = atomicAdd(&s_tail, 1);
-              wg_st(q + pos, w);
-              wg_st(del + w, g.w ? (double)g.w[w] : 0.0);
-              dw = d1;
-            } else {
-              dw = expect;
+        // arcs four at a time: their neighbour ids and depths load together
+        for (int64_t eb = e0; eb < e1; eb += 4) {
+          int32_t ws[4], ds[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) ws[t] = eb + t < e1 ? g.adj[eb + t] : -1;
+#pragma unroll
+          for (int t = 0; t < 4; t++) ds[t] = ws[t] >= 0 ? wg_ld(depth + ws[t]) : INT32_MAX;
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            if (ws[t] < 0) continue;
+            const int32_t w = ws[t];
+            int32_t dw = ds[t];
+            if (dw < 0) {
+              int32_t expect = -1;
+              if (__hip_atomic_compare_exchange_strong(depth + w, &expect, d1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                const int pos = atomicAdd(&s_tail, 1);
+                wg_st(q + pos, w);
+                wg_st(del + w, g.w ? (double)g.w[w] : 0.0);
+                dw = d1;
+              } else {
+                dw = expect;
+              }
             }
-          }
-          if (dw == d1) {
-            a_cnt++;
-            if (sc) {
-              __hip_atomic_fetch_add((double*)(sig + w), ub2d(sv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
-              u64 old = __hip_atomic_fetch_add(sig + w, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              if (old + sv < old) s_flag = 1;
+            if (dw == d1) {
+              a_cnt++;
+              if (sc) {
+                __hip_atomic_fetch_add((double*)(sig + w), ub2d(sv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              } else {
+                u64 old = __hip_atomic_fetch_add(sig + w, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (old + sv < old) s_flag = 1;
+              }
             }
           }
         }
@@ -1472,13 +1482,28 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
           const u64 sb = wg_ld(sig + v);
           const double sv = sc ? ub2d(sb) * inv_m : (double)sb;
           double acc = g.w ? (double)g.w[v] : 0.0;
-          for (int64_t e = g.off[v]; e < g.off[v + 1]; e++) {
-            const int32_t z = g.adj[e];
-            if (wg_ld(depth + z) == d + 1) {
-              const u64 zb = wg_ld(sig + z);
-              const double t = (wg_ld(del + z) + 1.0) / (sc ? ub2d(zb) : (double)zb);
-              acc = acc + t * sv;
+          const int64_t f0 = g.off[v], f1 = g.off[v + 1];
+          for (int64_t eb = f0; eb < f1; eb += 4) {
+            // four successors' loads in flight, the sum still in adjacency order
+            int32_t zs[4];
+            bool ok[4];
+            u64 zb[4];
+            double zd[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) zs[t] = eb + t < f1 ? g.adj[eb + t] : -1;
+#pragma unroll
+            for (int t = 0; t < 4; t++) ok[t] = zs[t] >= 0 && wg_ld(depth + zs[t]) == d + 1;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+              zb[t] = ok[t] ? wg_ld(sig + zs[t]) : 1ull;
+              zd[t] = ok[t] ? wg_ld(del + zs[t]) : 0.0;
             }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+              if (ok[t]) {
+                const double tt = (zd[t] + 1.0) / (sc ? ub2d(zb[t]) : (double)zb[t]);
+                acc = acc + tt * sv;
+              }
           }
           wg_st(del + v, acc);
           if (acc != 0.0) atomicAdd(a.bc + v, mult * acc);
@@ -2093,6 +2118,7 @@ struct Workspace {
   const DevGraph* helpers_graph = nullptr;
   // workgroup-per-source sweeps (deep graphs): per-slot state and per-run buffers
   int64_t wg_slots = 0, wg_lcap = 0;
+  int64_t wg_req = 0;                 // slots asked for (wg_slots may be fewer: device memory)
   int32_t* wg_depth = nullptr;
   u64* wg_sigma = nullptr;
   double* wg_delta = nullptr;
@@ -2718,14 +2744,24 @@ static int wg_alloc_slots(const DevGraph& g, Workspace* ws, const Params& prm, i
   ws->wg_depth = nullptr; ws->wg_sigma = nullptr; ws->wg_delta = nullptr; ws->wg_queue = nullptr;
   ws->wg_lvl = nullptr; ws->wg_scale = nullptr;
   const int64_t n = g.n;
-  const int64_t S = wg_slot_count(n, prm, lcap);
+  int64_t S = wg_slot_count(n, prm, lcap);
   size_t b = 0;
-  TRY(dalloc(&ws->wg_depth, (size_t)S * n, &b));
-  TRY(dalloc(&ws->wg_sigma, (size_t)S * n, &b));
-  TRY(dalloc(&ws->wg_delta, (size_t)S * n, &b));
-  TRY(dalloc(&ws->wg_queue, (size_t)S * n, &b));
-  TRY(dalloc(&ws->wg_lvl, (size_t)S * lcap, &b));
-  TRY(dalloc(&ws->wg_scale, (size_t)S * lcap, &b));
+  // fewer slots when the device cannot hold the requested per-source state
+  for (;;) {
+    b = 0;
+    int rc = dalloc(&ws->wg_depth, (size_t)S * n, &b);
+    if (rc == BCMI_OK) rc = dalloc(&ws->wg_sigma, (size_t)S * n, &b);
+    if (rc == BCMI_OK) rc = dalloc(&ws->wg_delta, (size_t)S * n, &b);
+    if (rc == BCMI_OK) rc = dalloc(&ws->wg_queue, (size_t)S * n, &b);
+    if (rc == BCMI_OK) rc = dalloc(&ws->wg_lvl, (size_t)S * lcap, &b);
+    if (rc == BCMI_OK) rc = dalloc(&ws->wg_scale, (size_t)S * lcap, &b);
+    if (rc == BCMI_OK) break;
+    fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_delta); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
+    ws->wg_depth = nullptr; ws->wg_sigma = nullptr; ws->wg_delta = nullptr; ws->wg_queue = nullptr;
+    ws->wg_lvl = nullptr; ws->wg_scale = nullptr;
+    if (S == 1) return rc;
+    S = (S + 1) / 2;
+  }
   if (!ws->wg_bc) {
     TRY(dalloc(&ws->wg_bc, n, &b));
     TRY(dalloc(&ws->wg_next, 1, &b));
@@ -2737,6 +2773,7 @@ static int wg_alloc_slots(const DevGraph& g, Workspace* ws, const Params& prm, i
   hipLaunchKernelGGL(k_fill_u64, grid_for(S * n), 256, 0, ws->st, ws->wg_sigma, 0ull, S * n);
   HIP_TRY(hipStreamSynchronize(ws->st));
   ws->wg_slots = S;
+  ws->wg_req = wg_slot_count(n, prm, lcap);
   ws->wg_lcap = lcap;
   ws->wg_bytes = b + ws->wg_cap * (sizeof(int32_t) * 3 + sizeof(double) + sizeof(int64_t) * 3);
   return BCMI_OK;
@@ -2765,7 +2802,7 @@ static int run_wg(const DevGraph& g, Workspace* ws, const Params& prm, const std
                   const std::vector<double>& cm, RunResult& R, int& ovf) {
   hipStream_t st = ws->st;
   const int64_t lcap0 = std::max(wg_initial_lcap(g), ws->wg_lcap);
-  if (!ws->wg_depth || ws->wg_lcap < lcap0 || ws->wg_slots != wg_slot_count(g.n, prm, lcap0))
+  if (!ws->wg_depth || ws->wg_lcap < lcap0 || ws->wg_req != wg_slot_count(g.n, prm, lcap0))
     TRY(wg_alloc_slots(g, ws, prm, lcap0));
   std::vector<int32_t> src(cs);
   std::vector<double> mult(cm);
