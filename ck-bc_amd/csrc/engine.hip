@@ -1318,8 +1318,8 @@ struct WA {
   int64_t nsrc;
   unsigned long long* next;  // next source index
   int32_t* depth;            // slot-major, n per slot; -1 between sweeps
-  u64* sigma;                // uint64 counts, or FP64 bits when scaled; 0 between sweeps
-  double* delta;
+  u64* sigma;                // uint64 counts, or FP64 bits when scaled; 0 between sweeps.
+                             // A finished vertex's slot holds coef = (delta+1)/sigma instead
   int32_t* queue;            // BFS order; level k = queue[lvl[k] .. lvl[k+1])
   int32_t* lvl;              // lcap per slot
   double* scale;             // scaled: per-level factor m(k), lcap per slot
@@ -1369,7 +1369,6 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
   const int64_t slot = blockIdx.x;
   int32_t* depth = a.depth + slot * n;
   u64* sig = a.sigma + slot * n;
-  double* del = a.delta + slot * n;
   int32_t* q = a.queue + slot * n;
   int32_t* lvl = a.lvl + slot * a.lcap;
   double* scl = a.scale + slot * a.lcap;
@@ -1387,7 +1386,6 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
     if (tid == 0) {
       wg_st(depth + s, 0);
       wg_st(sig + s, sc ? d2ub(1.0) : 1ull);
-      wg_st(del + s, 0.0);
       wg_st(q, s);
       wg_st(lvl, 0);
       wg_st(lvl + 1, 1);
@@ -1431,7 +1429,6 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
                                                        __HIP_MEMORY_SCOPE_WORKGROUP)) {
                 const int pos = atomicAdd(&s_tail, 1);
                 wg_st(q + pos, w);
-                wg_st(del + w, g.w ? (double)g.w[w] : 0.0);
                 dw = d1;
               } else {
                 dw = expect;
@@ -1472,6 +1469,8 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
     const int flag = s_flag;
     // Backward, deepest level first: delta(v) = w(v) + sum over successors z
     // of (delta(z)+1)/sigma(z)*sigma(v), in adjacency order (the oracle's).
+    // A finished vertex's slot then holds coef(v) = (delta(v)+1)/sigma(v),
+    // the quotient its predecessors use (computed once, same operations).
     if (!flag) {
       const double mult = a.mult[i];
       for (int d = L - 1; d >= 1; d--) {
@@ -1480,33 +1479,28 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
         for (int qq = lo + tid; qq < hi; qq += WGT) {
           const int32_t v = wg_ld(q + qq);
           const u64 sb = wg_ld(sig + v);
-          const double sv = sc ? ub2d(sb) * inv_m : (double)sb;
+          const double svr = sc ? ub2d(sb) : (double)sb;
+          const double sv = sc ? svr * inv_m : svr;
           double acc = g.w ? (double)g.w[v] : 0.0;
           const int64_t f0 = g.off[v], f1 = g.off[v + 1];
           for (int64_t eb = f0; eb < f1; eb += 4) {
             // four successors' loads in flight, the sum still in adjacency order
             int32_t zs[4];
             bool ok[4];
-            u64 zb[4];
-            double zd[4];
+            double zc[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) zs[t] = eb + t < f1 ? g.adj[eb + t] : -1;
 #pragma unroll
             for (int t = 0; t < 4; t++) ok[t] = zs[t] >= 0 && wg_ld(depth + zs[t]) == d + 1;
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-              zb[t] = ok[t] ? wg_ld(sig + zs[t]) : 1ull;
-              zd[t] = ok[t] ? wg_ld(del + zs[t]) : 0.0;
-            }
+            for (int t = 0; t < 4; t++) zc[t] = ok[t] ? ub2d(wg_ld(sig + zs[t])) : 0.0;
 #pragma unroll
             for (int t = 0; t < 4; t++)
-              if (ok[t]) {
-                const double tt = (zd[t] + 1.0) / (sc ? ub2d(zb[t]) : (double)zb[t]);
-                acc = acc + tt * sv;
-              }
+              if (ok[t]) acc = acc + zc[t] * sv;
           }
-          wg_st(del + v, acc);
           if (acc != 0.0) atomicAdd(a.bc + v, mult * acc);
+          // level d is read by level d-1 only after the barrier below
+          wg_st(sig + v, d2ub((acc + 1.0) / svr));
         }
         __syncthreads();
       }
@@ -2121,7 +2115,6 @@ struct Workspace {
   int64_t wg_req = 0;                 // slots asked for (wg_slots may be fewer: device memory)
   int32_t* wg_depth = nullptr;
   u64* wg_sigma = nullptr;
-  double* wg_delta = nullptr;
   int32_t* wg_queue = nullptr;
   int32_t* wg_lvl = nullptr;
   double* wg_scale = nullptr;
@@ -2222,7 +2215,7 @@ void workspace_destroy(Workspace* ws) {
   free_graphs(ws);
   {
     auto fw = [](void* p) { if (p) (void)hipFree(p); };
-    fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_delta); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
+    fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
     fw(ws->wg_bc); fw(ws->wg_next); fw(ws->wg_src); fw(ws->wg_mult); fw(ws->wg_st); fw(ws->wg_st2);
   }
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
@@ -2718,7 +2711,7 @@ static bool use_wg(const DevGraph& g, const Params& prm) {
 }
 
 static size_t wg_slot_bytes(int64_t n, int64_t lcap) {
-  return (size_t)n * (2 * sizeof(int32_t) + sizeof(u64) + sizeof(double)) +
+  return (size_t)n * (2 * sizeof(int32_t) + sizeof(u64)) +
          (size_t)lcap * (sizeof(int32_t) + sizeof(double));
 }
 
@@ -2740,8 +2733,8 @@ static size_t wg_bytes_estimate(const DevGraph& g, const Params& prm) {
 static int wg_alloc_slots(const DevGraph& g, Workspace* ws, const Params& prm, int64_t lcap) {
   ALLOC_LOCK;
   auto fw = [](void* p) { if (p) (void)hipFree(p); };
-  fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_delta); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
-  ws->wg_depth = nullptr; ws->wg_sigma = nullptr; ws->wg_delta = nullptr; ws->wg_queue = nullptr;
+  fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
+  ws->wg_depth = nullptr; ws->wg_sigma = nullptr; ws->wg_queue = nullptr;
   ws->wg_lvl = nullptr; ws->wg_scale = nullptr;
   const int64_t n = g.n;
   int64_t S = wg_slot_count(n, prm, lcap);
@@ -2751,13 +2744,12 @@ static int wg_alloc_slots(const DevGraph& g, Workspace* ws, const Params& prm, i
     b = 0;
     int rc = dalloc(&ws->wg_depth, (size_t)S * n, &b);
     if (rc == BCMI_OK) rc = dalloc(&ws->wg_sigma, (size_t)S * n, &b);
-    if (rc == BCMI_OK) rc = dalloc(&ws->wg_delta, (size_t)S * n, &b);
     if (rc == BCMI_OK) rc = dalloc(&ws->wg_queue, (size_t)S * n, &b);
     if (rc == BCMI_OK) rc = dalloc(&ws->wg_lvl, (size_t)S * lcap, &b);
     if (rc == BCMI_OK) rc = dalloc(&ws->wg_scale, (size_t)S * lcap, &b);
     if (rc == BCMI_OK) break;
-    fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_delta); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
-    ws->wg_depth = nullptr; ws->wg_sigma = nullptr; ws->wg_delta = nullptr; ws->wg_queue = nullptr;
+    fw(ws->wg_depth); fw(ws->wg_sigma); fw(ws->wg_queue); fw(ws->wg_lvl); fw(ws->wg_scale);
+    ws->wg_depth = nullptr; ws->wg_sigma = nullptr; ws->wg_queue = nullptr;
     ws->wg_lvl = nullptr; ws->wg_scale = nullptr;
     if (S == 1) return rc;
     S = (S + 1) / 2;
@@ -2817,7 +2809,7 @@ static int run_wg(const DevGraph& g, Workspace* ws, const Params& prm, const std
     hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, st, (u64*)ws->wg_next, 0ull, (int64_t)1);
     WA a;
     a.src = ws->wg_src; a.mult = ws->wg_mult; a.nsrc = k; a.next = ws->wg_next;
-    a.depth = ws->wg_depth; a.sigma = ws->wg_sigma; a.delta = ws->wg_delta; a.queue = ws->wg_queue;
+    a.depth = ws->wg_depth; a.sigma = ws->wg_sigma; a.queue = ws->wg_queue;
     a.lvl = ws->wg_lvl; a.scale = ws->wg_scale; a.lcap = ws->wg_lcap; a.bc = ws->wg_bc;
     a.st_n = ws->wg_st; a.st_m = ws->wg_st + ws->wg_cap; a.st_a = ws->wg_st + 2 * ws->wg_cap;
     a.st_levels = ws->wg_st2; a.st_flag = ws->wg_st2 + ws->wg_cap;
