@@ -158,8 +158,8 @@ __device__ __forceinline__ void kt_enter(KT* kt) {
 __device__ __forceinline__ void kt_exit(KT* kt) {
   if (kt && __lane_id() == 0) kt->wend[kt_wave()] = (u64)wall_clock64();
 }
-__global__ void __launch_bounds__(256) k_kt_accum(KT* kt, int64_t nwaves) {
-  __shared__ u64 s_lo[4], s_hi[4];
+__global__ void __launch_bounds__(1024) k_kt_accum(KT* kt, int64_t nwaves) {
+  __shared__ u64 s_lo[16], s_hi[16];
   u64 lo = ~0ull, hi = 0;
   for (int64_t i = threadIdx.x; i < nwaves; i += blockDim.x) {
     u64 a = kt->wstart[i], b = kt->wend[i];
@@ -2365,7 +2365,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
   auto tk = [&](int id) -> KT* { return kt == id ? T : nullptr; };
   // after a timed kernel of Gk blocks: fold its wave stamps into the total
   auto ktacc = [&](int id, int Gk) {
-    if (kt == id) hipLaunchKernelGGL(k_kt_accum, 1, 256, 0, st, T, (int64_t)Gk * B / 64);
+    if (kt == id) hipLaunchKernelGGL(k_kt_accum, 1, 1024, 0, st, T, (int64_t)Gk * B / 64);
   };
   // Four graphs: {large grid, small grid} x {forward, backward}.  The small
   // one serves levels with few vertices (long chains): its launches are cheap
