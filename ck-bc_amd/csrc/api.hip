@@ -112,6 +112,8 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "alpha_batched") {
     if (v < 0) return set_error(BCMI_ERR_ARG, "alpha must be >= 0");
     p.alpha_batched = v;
+  } else if (k == "fuse_bu") {
+    p.fuse_bu = v ? 1 : 0;
   } else if (k == "gamma_push") {
     if (v < 0) return set_error(BCMI_ERR_ARG, "gamma_push must be >= 0");
     p.gamma_push = v;
@@ -181,6 +183,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "beta_plain") *v = p.beta_plain;
   else if (k == "beta_batched") *v = p.beta_batched;
   else if (k == "gamma_push") *v = p.gamma_push;
+  else if (k == "fuse_bu") *v = p.fuse_bu;
   else if (k == "steps") *v = p.steps;
   else if (k == "width") *v = p.width;
   else if (k == "ktime") *v = p.ktime;

@@ -29,6 +29,7 @@ struct Params {
   int64_t alpha_batched = 1000; // x1000
   int64_t beta_plain = 16;      // bottom-up only when frontier arcs >= all arcs / beta
   int64_t beta_batched = 16;
+  int fuse_bu = 1;              // bottom-up levels sum path counts of unsplit vertices in the same scan
   int64_t gamma_push = 8;       // batched top-down levels with frontier arcs < all arcs / gamma push
                                 // path counts from the frontier (0: always pull from parents)
   int steps = 16;           // levels per captured graph

@@ -248,6 +248,7 @@ struct BA {
   double* dbg_delta;
   int64_t alpha;
   int64_t beta;   // bottom-up only when the frontier holds >= 1/beta of all arcs
+  int fuse;       // bottom-up levels also sum the path counts of unsplit vertices
   int64_t gamma;  // top-down levels whose frontier holds < 1/gamma of all arcs push path counts
   int64_t n;
   double* lscale;      // [level][64] scale factor m of each level (scaled path counts)
@@ -885,6 +886,12 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 
 // Bottom-up discovery: every vertex not yet reached by all sources of its
 // component ORs the level-d masks of its neighbours.
+// Bottom-up discovery: every vertex not yet reached by all sources of its
+// component ORs the level-d masks of its neighbours.  With fusion on, a
+// vertex held in one record also sums its parents' path counts in the same
+// arc scan (lanes = sources, parents in adjacency order, as k_ms_sigma
+// would), so the sigma kernel skips its parent scan for that vertex.
+#define UNR_BU 4
 __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone || c->mode != 1) return;
@@ -896,6 +903,12 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   const int lane = lane_id();
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  const bool fuse = b.fuse;
+  const bool scaled = c->scaled;
+  const double inv_m = (fuse && scaled) ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
+  u64 ca = 0;
+  int ovf = 0;
+  double mx = 0.0;
   APPENDER(ap);
   for (int64_t r = gw; r < g.nall; r += tw) {
     int32_t v = g.all_v[r];
@@ -903,12 +916,54 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
     u64 tgt = b.compmask[g.label[v]];
     if (sv == tgt) continue;
     int j = g.all_j[r];
+    const int64_t deg = g.off[v + 1] - g.off[v];
     int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
     int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
-    u64 acc = 0;
+    const bool fr = fuse && deg <= g.chunk;
+    u64 acc = 0, sacc = 0;
+    double facc = 0.0;
     for (int64_t eb = e0; eb < e1; eb += 64) {
       int64_t e = eb + lane;
-      if (e < e1) acc |= cur[g.adj[e]];
+      int32_t u = 0;
+      u64 F = 0;
+      if (e < e1) {
+        u = g.adj[e];
+        F = cur[u];
+        acc |= F;
+      }
+      if (!fr) continue;
+      const u64 need = F & ~sv;  // u is a parent of v for these sources
+      u64 P = __ballot(need != 0);
+      while (P) {
+        int q[UNR_BU];
+        bool okq[UNR_BU];
+#pragma unroll
+        for (int t = 0; t < UNR_BU; t++) {
+          okq[t] = P != 0;
+          q[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
+          if (okq[t]) P &= P - 1;
+        }
+        u64 x[UNR_BU];
+#pragma unroll
+        for (int t = 0; t < UNR_BU; t++) {
+          const int32_t uq = rl32(u, q[t]);
+          const u64 nq = rl64(need, q[t]);
+          const bool use = okq[t] && ((nq >> lane) & 1);
+          x[t] = use ? b.sigma[(int64_t)uq * 64 + lane] : 0;
+          ca += use;
+        }
+        if (scaled) {
+#pragma unroll
+          for (int t = 0; t < UNR_BU; t++) facc = facc + __longlong_as_double((long long)x[t]);
+        } else {
+#pragma unroll
+          for (int t = 0; t < UNR_BU; t++) {
+            u64 s2 = sacc + x[t];
+            if (s2 < sacc) ovf = 1;
+            sacc = s2;
+          }
+        }
+      }
     }
     acc = wave_or64(acc) & ~sv;
     bool won = false;
@@ -918,15 +973,30 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         u64 old = atomicOr(&nxt[v], acc);
         won = (old == 0);
         if (won) {
-          dg = g.off[v + 1] - g.off[v];
+          dg = deg;
           if (g.flags & KF_GATE) lb_set(nlb, v);
         }
+      }
+    }
+    if (fr && ((acc >> lane) & 1)) {
+      u64* row = b.sigma + (int64_t)v * 64 + lane;
+      if (scaled) {
+        const double raw = facc * inv_m;
+        *row = (u64)__double_as_longlong(raw);
+        mx = fmax(mx, raw);
+      } else {
+        *row = sacc;
       }
     }
     app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
     zero_multi_sigma(won && dg > g.chunk, v, b);
   }
   app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
+  if (fuse) {
+    block_add_lanes(ca, b.st_a);
+    if (scaled) block_max_lanes((u64)__double_as_longlong(mx), b.lvmax);
+    if (ovf) c->sigma_ovf = 1;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
@@ -955,7 +1025,8 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   int ovf = 0;
   long long done_arcs = 0, uv = 0, ua = 0;
   const bool scaled = c->scaled;
-  const bool push = c->push;  // the top-down kernel already summed the parents
+  const bool push = c->push;  // k_ms_push sums the parents
+  const bool fused = c->mode == 1 && b.fuse;  // k_ms_bu summed the parents of unsplit vertices
   const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   double mx = 0.0;
   for (int64_t r = lo + gw; r < hi; r += tw) {
@@ -965,7 +1036,8 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
     int64_t deg = g.off[v + 1] - g.off[v];
     int nch = nchunks(deg, g.chunk);
     int64_t e0, e1;
-    if (push) {
+    const bool done_by_bu = fused && nch == 1;
+    if (push || done_by_bu) {
       e0 = e1 = 0;  // no parent scan
     } else if (!g.directed) {
       e0 = g.off[v] + (int64_t)j * g.chunk;
@@ -1021,7 +1093,9 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
       }
     }
     u64* row = b.sigma + (int64_t)v * 64 + lane;
-    if (push) {
+    if (done_by_bu) {
+      // row written by k_ms_bu
+    } else if (push) {
       if (mine && j == 0) *row = 0ull;  // k_ms_push adds the parents' counts next
     } else if (scaled) {
       double raw = accf * inv_m;
@@ -2153,7 +2227,7 @@ static BA make_ba(Workspace* ws, const Params& prm) {
   b.rec_v = ws->rec_v; b.rec_j = ws->rec_j; b.rec_mask = ws->rec_mask; b.cap = ws->cap; b.rec_lo = ws->rec_lo;
   b.compmask = ws->compmask; b.srcbuf = ws->srcbuf; b.multbuf = ws->multbuf; b.bc = ws->bc;
   b.st_n = ws->stats; b.st_m = ws->stats + 64; b.st_a = ws->stats + 128; b.dbg_delta = ws->dbg_delta;
-  b.alpha = prm.alpha_batched; b.beta = prm.beta_batched; b.gamma = prm.gamma_push; b.n = ws->n;
+  b.alpha = prm.alpha_batched; b.beta = prm.beta_batched; b.gamma = prm.gamma_push; b.fuse = prm.fuse_bu; b.n = ws->n;
   b.lscale = ws->lscale; b.lscale_cap = ws->lscale_cap; b.lvmax = ws->lvmax;
   return b;
 }
@@ -2427,7 +2501,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
 static bool same_launch(const Params& a, const Params& b) {
   return a.block == b.block && a.grid == b.grid && a.chunk == b.chunk && a.alpha_plain == b.alpha_plain &&
          a.alpha_batched == b.alpha_batched && a.beta_plain == b.beta_plain && a.beta_batched == b.beta_batched &&
-         a.gamma_push == b.gamma_push &&
+         a.gamma_push == b.gamma_push && a.fuse_bu == b.fuse_bu &&
          a.steps == b.steps && a.ktime == b.ktime &&
          a.kflags == b.kflags && a.sgrid == b.sgrid;
 }

@@ -123,6 +123,7 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * work item), "alpha_plain"/"alpha_batched" (x1000: top-down -> bottom-up
  * switch when frontier arcs > alpha * unvisited arcs), "beta_plain"/
  * "beta_batched" (and only when frontier arcs >= all arcs / beta),
+ * "fuse_bu" (1: bottom-up levels also sum path counts in their scan),
  * "gamma_push" (batched top-down levels whose frontier holds fewer than
  * all arcs / gamma push path counts from the frontier; 0 always pulls),
  * "steps" (levels per
