@@ -1358,21 +1358,31 @@ __global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b) {
   }
 }
 
+// Resets the batch state for the next sweep.  seen is cleared for every
+// reached vertex and compmask for the sources' components.  After a complete
+// backward pass only levels 1 and 2 are left in the level masks (each step
+// clears the level two below it), so only their records clear lm; a sweep
+// whose backward pass was skipped clears lm for every record.
 __global__ void k_ms_cleanup(GA g, BA b) {
   Ctl* c = b.ctl;
   if (!c->fdone || !c->bdone) return;
   int64_t n = c->rec_count < b.cap ? c->rec_count : b.cap;
+  const bool full = c->overflow || (c->sigma_ovf && !c->scaled) || (g.flags & KF_GATE);
+  const int64_t src_hi = b.rec_lo[1];
+  const int64_t lm_hi = full ? n : b.rec_lo[min(3, c->last + 1)];
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     if (b.rec_j[r] == 0) {
       int32_t v = b.rec_v[r];
       b.seen[v] = 0;
-      b.lm[v] = 0;
-      b.lm[b.n + v] = 0;
-      b.lm[2 * b.n + v] = 0;
+      if (r < lm_hi) {
+        b.lm[v] = 0;
+        b.lm[b.n + v] = 0;
+        b.lm[2 * b.n + v] = 0;
+      }
       if (g.flags & KF_GATE) lb_clr(b.lb, v);
       if (g.flags & KF_GATE) lb_clr(b.lb + b.nw, v);
       if (g.flags & KF_GATE) lb_clr(b.lb + 2 * b.nw, v);
-      b.compmask[g.label[v]] = 0;
+      if (r < src_hi) b.compmask[g.label[v]] = 0;
     }
   }
 }
