@@ -4,7 +4,7 @@ Coordinate descent on measured throughput (GTEPS of a sample of batches):
 block shape, grid, record size (edges per work item), the top-down ->
 bottom-up switch threshold (batched and plain), the frontier floor below
 which bottom-up is not used, the frontier size below which path counts are
-pushed from the frontier, and the batch width.  (The
+pushed from the frontier, the levels per captured graph, and the batch width.  (The
 small-grid size for sparse levels is left at the engine default, which a
 sweep on bridged-block chains set; R-MAT graphs do not depend on it.)  Results go to a JSON table keyed by a
 graph class ("rmat-s22-ef16", "file:<name>", ...) that bench.py and callers
@@ -30,10 +30,11 @@ SPACE = {
     "alpha_batched": [250, 500, 1000, 2000, 4000, 1 << 60],
     "beta_batched": [4, 8, 16, 32, 64],
     "gamma_push": [0, 4, 8, 16, 32],
+    "steps": [8, 10, 12, 16],
     "width": [32, 48, 64],
 }
 PLAIN_SPACE = {"alpha_plain": [100, 300, 1000, 3000, 10000, 1 << 60]}
-CHUNKS = [128, 256, 512]
+CHUNKS = [256, 512, 1024]
 DEFAULT_PATH = os.path.join(HERE, "tuned.json")
 
 
@@ -56,14 +57,14 @@ def measure(e, src, mode, reps=1):
 
 def tune(g, src_batched, src_plain, log=print):
     params = {"chunk": 256, "block": 256, "grid": 4096, "alpha_batched": 1000, "beta_batched": 16, "gamma_push": 8,
-              "width": 64,
+              "steps": 16, "width": 64,
               "alpha_plain": 1000}
     best_overall = (-1.0, None)
     for ch in CHUNKS:
         e = bcmi.Engine(0, chunk=ch)
         e.upload(g)
         cur = dict(params, chunk=ch)
-        for k in ("block", "grid", "alpha_batched", "beta_batched", "gamma_push", "width"):
+        for k in ("block", "grid", "alpha_batched", "beta_batched", "gamma_push", "steps", "width"):
             if k != "chunk":
                 e.set_param(k, cur[k])
         e.run(src_batched[:64])  # capture
@@ -114,7 +115,8 @@ def apply(engine, table, key):
     p = table.get(key) or table.get("default")
     if not p:
         return None
-    for k in ("chunk", "block", "grid", "alpha_batched", "beta_batched", "gamma_push", "alpha_plain", "width"):
+    for k in ("chunk", "block", "grid", "alpha_batched", "beta_batched", "gamma_push", "steps", "alpha_plain",
+              "width"):
         if k in p:
             engine.set_param(k, int(p[k]))
     return p
@@ -133,7 +135,7 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20250101)
-    ap.add_argument("--batches", type=int, default=6, help="batches of 64 sources per measurement")
+    ap.add_argument("--batches", type=int, default=12, help="batches of 64 sources per measurement")
     ap.add_argument("--plain", type=int, default=16, help="sources per plain measurement")
     ap.add_argument("--out", default=DEFAULT_PATH)
     a = ap.parse_args()
