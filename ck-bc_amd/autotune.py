@@ -33,7 +33,7 @@ SPACE = {
     "steps": [8, 10, 12, 16],
     "width": [32, 48, 64],
 }
-PLAIN_SPACE = {"alpha_plain": [100, 300, 1000, 3000, 10000, 1 << 60]}
+PLAIN_SPACE = {"alpha_plain": [100, 300, 1000, 3000, 10000, 1 << 60], "grid_plain": [2048, 4096, 8192]}
 CHUNKS = [256, 512, 1024]
 DEFAULT_PATH = os.path.join(HERE, "tuned.json")
 
@@ -56,7 +56,7 @@ def measure(e, src, mode, reps=1):
 
 
 def tune(g, src_batched, src_plain, log=print):
-    params = {"chunk": 256, "block": 256, "grid": 4096, "alpha_batched": 1000, "beta_batched": 16, "gamma_push": 8,
+    params = {"chunk": 256, "block": 256, "grid": 4096, "grid_plain": 4096, "alpha_batched": 1000, "beta_batched": 16, "gamma_push": 8,
               "steps": 16, "width": 64,
               "alpha_plain": 1000}
     best_overall = (-1.0, None)
@@ -93,15 +93,18 @@ def tune(g, src_batched, src_plain, log=print):
             score = best_s
         log(f"chunk {ch}: best {score:.1f} GTEPS with {cur}")
         if score > best_overall[0]:
-            # plain threshold on this engine
-            pv, ps = cur["alpha_plain"], -1.0
-            for v in PLAIN_SPACE["alpha_plain"]:
-                e.set_param("alpha_plain", v)
-                s = measure(e, src_plain, bcmi.MODE_PLAIN)
-                log(f"  plain alpha_plain={v}: {s:.2f} GTEPS")
-                if s > ps:
-                    pv, ps = v, s
-            cur["alpha_plain"] = pv
+            # plain sweep parameters on this engine (coordinate descent)
+            ps = measure(e, src_plain, bcmi.MODE_PLAIN)
+            for pk, vals in PLAIN_SPACE.items():
+                pv = cur.get(pk, vals[0])
+                for v in vals:
+                    e.set_param(pk, v)
+                    s = measure(e, src_plain, bcmi.MODE_PLAIN)
+                    log(f"  plain {pk}={v}: {s:.2f} GTEPS")
+                    if s > ps:
+                        pv, ps = v, s
+                e.set_param(pk, pv)
+                cur[pk] = pv
             cur["plain_gteps"] = round(ps, 3)
             cur["batched_gteps"] = round(score, 2)
             best_overall = (score, dict(cur))
@@ -115,8 +118,8 @@ def apply(engine, table, key):
     p = table.get(key) or table.get("default")
     if not p:
         return None
-    for k in ("chunk", "block", "grid", "alpha_batched", "beta_batched", "gamma_push", "steps", "alpha_plain",
-              "width"):
+    for k in ("chunk", "block", "grid", "grid_plain", "alpha_batched", "beta_batched", "gamma_push", "steps",
+              "alpha_plain", "width"):
         if k in p:
             engine.set_param(k, int(p[k]))
     return p

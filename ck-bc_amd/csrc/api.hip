@@ -102,6 +102,9 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "grid") {
     if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "grid must be in [1,65536]");
     p.grid = (int)v;
+  } else if (k == "grid_plain") {
+    if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "grid_plain must be in [1,65536]");
+    p.grid_plain = (int)v;
   } else if (k == "chunk") {
     if (v < 64 || v > (1 << 20) || v % 64) return set_error(BCMI_ERR_ARG, "chunk must be a multiple of 64");
     if (c->have_graph) return set_error(BCMI_ERR_STATE, "chunk must be set before uploading a graph");
@@ -177,6 +180,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   const Params& p = c->prm;
   if (k == "block") *v = p.block;
   else if (k == "grid") *v = p.grid;
+  else if (k == "grid_plain") *v = p.grid_plain;
   else if (k == "chunk") *v = p.chunk;
   else if (k == "alpha_plain") *v = p.alpha_plain;
   else if (k == "alpha_batched") *v = p.alpha_batched;

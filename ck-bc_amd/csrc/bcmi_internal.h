@@ -23,7 +23,8 @@ void relabel_by_degree(int64_t n, const int64_t* off, const int32_t* adj, std::v
 // Tunable launch and algorithm parameters.
 struct Params {
   int block = 256;          // threads per block (multiple of 64)
-  int grid = 4096;          // blocks per expansion kernel (about twice what is resident)
+  int grid = 4096;          // blocks per batched expansion kernel (about twice what is resident)
+  int grid_plain = 4096;    // blocks per plain-sweep kernel
   int chunk = 256;          // max edges per work item
   int64_t alpha_plain = 1000;   // x1000; bottom-up when frontier arcs > alpha * unvisited arcs
   int64_t alpha_batched = 1000; // x1000

@@ -2457,7 +2457,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
   for (int which = 0; which < 4; which++) {
     const bool fwd = (which & 1) == 0;
     const bool sparse = which >= 2;
-    const int Gk = sparse ? prm.sgrid : G;
+    const int Gk = sparse ? prm.sgrid : plain ? prm.grid_plain : G;
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < prm.steps; s++) {
@@ -2509,7 +2509,8 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
 }
 
 static bool same_launch(const Params& a, const Params& b) {
-  return a.block == b.block && a.grid == b.grid && a.chunk == b.chunk && a.alpha_plain == b.alpha_plain &&
+  return a.block == b.block && a.grid == b.grid && a.grid_plain == b.grid_plain && a.chunk == b.chunk &&
+         a.alpha_plain == b.alpha_plain &&
          a.alpha_batched == b.alpha_batched && a.beta_plain == b.beta_plain && a.beta_batched == b.beta_batched &&
          a.gamma_push == b.gamma_push && a.fuse_bu == b.fuse_bu &&
          a.steps == b.steps && a.ktime == b.ktime &&
