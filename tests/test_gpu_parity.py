@@ -266,3 +266,34 @@ def test_batched_delta_bit_exact_when_no_vertex_is_split(reorder):
             assert np.array_equal(dm[q], od) and np.array_equal(sgm[q], osg)
             assert np.array_equal(dlm[q], odl), q
     e2.close()
+
+
+@pytest.mark.parametrize("strategy", [1, 2])
+@pytest.mark.parametrize("mode", [bcmi.MODE_PLAIN, bcmi.MODE_BATCHED])
+def test_degenerate_inputs(mode, strategy):
+    """No edges, no sources, degree-0 sources only, a single edge, and a
+    star with peeling: BC is what the oracle says (zero where no pair has an
+    inner vertex), in every engine."""
+    e = bcmi.Engine(strategy=strategy)
+    g = bcmi.Graph.from_edges(np.zeros((0, 2), np.int32), 5)
+    e.upload(g)
+    e.reset()
+    st = e.run(np.arange(5, dtype=np.int32), mode=mode)
+    assert st["sources_run"] == 0 and np.all(e.bc() == 0)
+    k2 = bcmi.Graph.from_edges([(0, 1)], 3)
+    for h in (bcmi.H_NONE, bcmi.H_PEEL):
+        e.upload(k2, heuristic=h)
+        e.reset()
+        e.run(np.array([], np.int32), mode=mode)
+        assert np.all(e.bc() == 0)
+        e.run(np.array([0, 1, 2, 1], np.int32), mode=mode)
+        assert np.all(e.bc() == 0)
+    star = bcmi.Graph.from_edges(*G.star(9))
+    for h in (bcmi.H_NONE, bcmi.H_PEEL):
+        e.upload(star, heuristic=h)
+        e.reset()
+        e.run(np.arange(star.n, dtype=np.int32), mode=mode)
+        ref = O.brandes(star.off, star.adj, np.arange(star.n))
+        assert bcmi.rel_err(e.bc(), ref) <= BC_TOL
+        assert e.bc()[0] == 9 * 8                      # ordered pairs through the centre
+    e.close()
