@@ -5,7 +5,9 @@
 // slots and HBM budget free up; each admitted query runs on its own worker
 // thread and HIP stream, so the sweeps of different queries overlap on the
 // device.  A query's result depends only on its own sources: it is computed
-// in its own workspace, batch after batch, in a fixed order.
+// in its own workspace, batch after batch (the level-loop engines sum in a
+// fixed order; the workgroup engine of deep graphs adds with atomics, whose
+// order changes only the last bits).
 //
 // No counterpart in the reference (one program run per graph, bc2d.c:673).
 #include <hip/hip_runtime.h>
