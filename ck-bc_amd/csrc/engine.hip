@@ -891,7 +891,7 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 // vertex held in one record also sums its parents' path counts in the same
 // arc scan (lanes = sources, parents in adjacency order, as k_ms_sigma
 // would), so the sigma kernel skips its parent scan for that vertex.
-#define UNR_BU 4
+#define UNR_BU 8
 __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone || c->mode != 1) return;
