@@ -30,9 +30,12 @@ struct Params {
   int64_t alpha_batched = 1000; // x1000
   int64_t beta_plain = 16;      // bottom-up only when frontier arcs >= all arcs / beta
   int64_t beta_batched = 16;
-  int fuse_bu = 1;              // bottom-up levels sum path counts of unsplit vertices in the same scan
+  int fuse_bu = 2;              // bottom-up levels sum path counts in the same scan: 0 never, 1 of
+                                // unsplit vertices, 2 of all (split vertices add per record atomically)
   int64_t gamma_push = 8;       // batched top-down levels with frontier arcs < all arcs / gamma push
                                 // path counts from the frontier (0: always pull from parents)
+  int64_t gamma_back = 0;       // batched backward levels whose successor level holds fewer than
+                                // 1/gamma_back of their arcs push coef from it (0: always pull)
   int steps = 16;           // levels per captured graph
   int sgrid = 512;          // blocks per kernel in the small-grid (sparse-level) graphs
   int64_t sparse_verts = 16384; // levels below this many vertices replay the small-grid graphs

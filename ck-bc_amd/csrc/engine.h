@@ -28,6 +28,8 @@ struct Ctl {
   int32_t width;      // sources in the batch
   int32_t scaled;     // batched: path counts held as FP64 scaled per level
   int32_t push;       // batched: this level's path counts are pushed from the frontier (k_ms_push)
+  int32_t bpush;      // batched: this backward level pulls nothing; the successor level pushes (k_ms_bpush)
+  int32_t pad0;
   int64_t rec_count;  // records appended so far
   int64_t arcs_next;  // degree sum of vertices appended during this step
   int64_t verts_next;
@@ -62,6 +64,8 @@ struct DevGraph {
   int32_t* all_v = nullptr;     // chunk records of every traversed vertex
   int32_t* all_j = nullptr;
   int64_t nall = 0;
+  int32_t* split_v = nullptr;   // vertices held in more than one chunk record
+  int64_t nsplit = 0;
   // peeling
   int32_t nrounds = 0;
   int32_t* round = nullptr;

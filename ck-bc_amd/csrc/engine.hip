@@ -195,6 +195,8 @@ struct GA {
   const int32_t* all_v;
   const int32_t* all_j;
   int64_t nall;
+  const int32_t* split_v;  // vertices held in more than one record
+  int64_t nsplit;
   const uint64_t* w;
   int chunk;
   int64_t n;
@@ -250,6 +252,8 @@ struct BA {
   int64_t beta;   // bottom-up only when the frontier holds >= 1/beta of all arcs
   int fuse;       // bottom-up levels also sum the path counts of unsplit vertices
   int64_t gamma;  // top-down levels whose frontier holds < 1/gamma of all arcs push path counts
+  int64_t gamma_b;     // backward levels whose successor level holds < 1/gamma_b of their arcs are pushed
+  int64_t* lvl_arcs;   // [level] degree sum of the level's vertices
   int64_t n;
   double* lscale;      // [level][64] scale factor m of each level (scaled path counts)
   int64_t lscale_cap;  // levels it can hold
@@ -626,6 +630,28 @@ __global__ void k_bdecide(Ctl* c) {
   if (c->blevel < 1) c->bdone = 1;
 }
 
+// Batched backward decide: as k_bdecide, and on undirected graphs a level
+// whose successor level holds fewer than 1/gamma_b of its arcs is pushed
+// (k_ms_bpush scans the successors' arcs instead of the level's own).
+__global__ void k_ms_bdecide(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (!c->fdone || c->bdone) return;
+  if (c->overflow || (c->sigma_ovf && !c->scaled)) {
+    c->bdone = 1;
+    return;
+  }
+  if (!c->bstarted) {
+    c->blevel = c->last;
+    c->bstarted = 1;
+  } else {
+    c->blevel--;
+  }
+  const int d = c->blevel;
+  if (d < 1) c->bdone = 1;
+  c->bpush = d >= 1 && d < c->last && b.gamma_b > 0 && !g.directed &&
+             (double)b.lvl_arcs[d + 1] * (double)b.gamma_b < (double)b.lvl_arcs[d];
+}
+
 // Backward level d: delta[v] = w(v) + sum over successors z of
 // (delta[z]+1)/sigma[z]*sigma[v]; coef[z] = (delta[z]+1)/sigma[z] was stored
 // when z was finished.  Vertices split over several records combine their
@@ -726,7 +752,7 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled) {
     c->arcs_next = 0; c->verts_next = 0; c->arcs_cur = 0; c->verts_cur = 0;
     c->arcs_seen = 0; c->sum_a = 0; c->n_td = 0; c->n_bu = 0;
     c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 0; c->uni_arcs = 0;
-    c->sigma_ovf = 0; c->width = k; c->scaled = scaled; c->push = 0;
+    c->sigma_ovf = 0; c->width = k; c->scaled = scaled; c->push = 0; c->bpush = 0;
     b.rec_lo[0] = 0;
   }
   b.st_n[i] = 0; b.st_m[i] = 0; b.st_a[i] = 0;
@@ -791,6 +817,7 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled) {
 __global__ void k_ms_decide(GA g, BA b, int allow_bu) {
   Ctl* c = b.ctl;
   if (!decide_common(c, b.rec_lo, b.alpha)) return;
+  b.lvl_arcs[c->level] = c->arcs_cur;
   if (c->scaled) {
     // Scaled path counts: level d holds raw(v) = sigma(v)/S(d-1); its factor
     // m(d) (the largest raw value, 1 if none) gives S(d) = S(d-1) m(d).
@@ -817,21 +844,6 @@ __global__ void k_ms_decide(GA g, BA b, int allow_bu) {
   // Small top-down frontiers push path counts along the arcs they expand
   // (the next level's own arcs, many more, are then not scanned for parents).
   c->push = !bu && b.gamma > 0 && (double)c->arcs_cur * (double)b.gamma < (double)g.m;
-}
-
-// Zeroes the sigma columns of the sources that have not reached v yet, for
-// vertices split over several records (their records add into sigma
-// atomically).  Wave-wide.
-__device__ __forceinline__ void zero_multi_sigma(bool need, int32_t v, const BA& b) {
-  u64 pend = __ballot(need);
-  const int lane = lane_id();
-  while (pend) {
-    int q = __ffsll((long long)pend) - 1;
-    pend &= pend - 1;
-    int32_t vq = rl32(v, q);
-    u64 sn = b.seen[vq];
-    if (!((sn >> lane) & 1)) b.sigma[(int64_t)vq * 64 + lane] = 0;
-  }
 }
 
 // Top-down discovery: for every arc u->v out of level d, the sources of u
@@ -872,7 +884,6 @@ __device__ __forceinline__ void k_ms_td_body(GA g, BA b) {
         }
       }
       app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
-      zero_multi_sigma(won && dg > g.chunk && !c->push, v, b);
     }
   }
   app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
@@ -885,12 +896,12 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 }
 
 // Bottom-up discovery: every vertex not yet reached by all sources of its
-// component ORs the level-d masks of its neighbours.
-// Bottom-up discovery: every vertex not yet reached by all sources of its
-// component ORs the level-d masks of its neighbours.  With fusion on, a
-// vertex held in one record also sums its parents' path counts in the same
-// arc scan (lanes = sources, parents in adjacency order, as k_ms_sigma
-// would), so the sigma kernel skips its parent scan for that vertex.
+// component ORs the level-d masks of its neighbours.  With fusion on, each
+// record also sums its parents' path counts in the same arc scan (lanes =
+// sources, parents in adjacency order, as k_ms_sigma would) and the sigma
+// kernel skips its parent scan: a vertex held in one record stores its sum,
+// the records of a split vertex (fuse_bu 2) add theirs atomically into its
+// row, whose columns are zero until the source reaches it.
 #define UNR_BU 8
 __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   Ctl* c = b.ctl;
@@ -919,7 +930,8 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
     const int64_t deg = g.off[v + 1] - g.off[v];
     int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
     int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
-    const bool fr = fuse && deg <= g.chunk;
+    const bool split = deg > g.chunk;
+    const bool fr = fuse && (!split || b.fuse > 1);
     u64 acc = 0, sacc = 0;
     double facc = 0.0;
     for (int64_t eb = e0; eb < e1; eb += 64) {
@@ -982,14 +994,22 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       u64* row = b.sigma + (int64_t)v * 64 + lane;
       if (scaled) {
         const double raw = facc * inv_m;
-        *row = (u64)__double_as_longlong(raw);
-        mx = fmax(mx, raw);
-      } else {
+        if (!split) {
+          *row = (u64)__double_as_longlong(raw);
+          mx = fmax(mx, raw);
+        } else {
+          // every returned sum is a lower bound of a final one; the last is final
+          const double o = atomicAdd((double*)row, raw);
+          mx = fmax(mx, o + raw);
+        }
+      } else if (!split) {
         *row = sacc;
+      } else {
+        const u64 o = atomicAdd(row, sacc);
+        if (o + sacc < o) ovf = 1;
       }
     }
     app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
-    zero_multi_sigma(won && dg > g.chunk, v, b);
   }
   app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
   if (fuse) {
@@ -1026,7 +1046,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   long long done_arcs = 0, uv = 0, ua = 0;
   const bool scaled = c->scaled;
   const bool push = c->push;  // k_ms_push sums the parents
-  const bool fused = c->mode == 1 && b.fuse;  // k_ms_bu summed the parents of unsplit vertices
+  const bool fused = c->mode == 1 && b.fuse;  // k_ms_bu summed the parents (of unsplit vertices if fuse 1)
   const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   double mx = 0.0;
   for (int64_t r = lo + gw; r < hi; r += tw) {
@@ -1036,7 +1056,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
     int64_t deg = g.off[v + 1] - g.off[v];
     int nch = nchunks(deg, g.chunk);
     int64_t e0, e1;
-    const bool done_by_bu = fused && nch == 1;
+    const bool done_by_bu = fused && (nch == 1 || b.fuse > 1);
     if (push || done_by_bu) {
       e0 = e1 = 0;  // no parent scan
     } else if (!g.directed) {
@@ -1239,9 +1259,22 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   // level d and w on level d+1, so the terms use raw(v)/m(d) and coef keeps raw.
   const bool scaled = c->scaled;
   const double inv_md = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
+  const bool bpush = c->bpush;
   for (int64_t r = lo + gw; r < hi; r += tw) {
     int32_t v = b.rec_v[r];
     int j = b.rec_j[r];
+    if (bpush) {
+      // pushed level: only zero the accumulators and publish the level
+      if (j == 0) {
+        u64 L = b.rec_mask[r];
+        if ((L >> lane) & 1) b.coef[(int64_t)v * 64 + lane] = 0.0;
+        if (lane == 0) {
+          lmd[v] = L;
+          if (g.flags & KF_GATE) lb_set(dlb, v);
+        }
+      }
+      continue;
+    }
     u64 L = b.rec_mask[r - j];
     int64_t deg = g.off[v + 1] - g.off[v];
     int nch = nchunks(deg, g.chunk);
@@ -1325,8 +1358,58 @@ __global__ void __launch_bounds__(256) k_ms_back(GA g, BA b, KT* kt) {
   kt_exit(kt);
 }
 
-// Finishes the level-d vertices whose arcs were split over several records:
-// delta = w(v) + the atomically added partial sums held in coef.
+// Pushed backward level d (undirected): for every arc z-u out of a level-d+1
+// record, coef[u][i] += coef[z][i] for the sources i with z on level d+1 and
+// u on level d.  k_ms_bfin then multiplies the sums by sigma[u][i].  The
+// successor level's arcs are far fewer than level d's, which the pull in
+// k_ms_back would scan; the sums are added in arrival order.
+__device__ __forceinline__ void k_ms_bpush_body(GA g, BA b) {
+  Ctl* c = b.ctl;
+  if (!c->fdone || c->bdone || !c->bpush) return;
+  const int d = c->blevel;
+  const u64* lmd = b.lm + (int64_t)(d % 3) * b.n;
+  const unsigned* dlb = b.lb + (int64_t)(d % 3) * b.nw;
+  const int64_t lo = b.rec_lo[d + 1], hi = b.rec_lo[d + 2];
+  const int lane = lane_id();
+  const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
+  for (int64_t r = lo + gw; r < hi; r += tw) {
+    const int32_t z = b.rec_v[r];
+    const int j = b.rec_j[r];
+    const u64 F = b.rec_mask[r - j];
+    const int64_t e0 = g.off[z] + (int64_t)j * g.chunk;
+    const int64_t e1 = min(g.off[z + 1], e0 + g.chunk);
+    const double cz = ((F >> lane) & 1) ? b.coef[(int64_t)z * 64 + lane] : 0.0;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      const int64_t e = eb + lane;
+      int32_t u = 0;
+      u64 M = 0;
+      if (e < e1) {
+        u = adj_at(g, g.adj, e);
+        if (!(g.flags & KF_GATE) || lb_test(dlb, u)) M = lmd[u] & F;
+      }
+      u64 P = __ballot(M != 0);
+      while (P) {
+        const int q = __ffsll((long long)P) - 1;
+        P &= P - 1;
+        const int32_t uq = rl32(u, q);
+        const u64 mq = rl64(M, q);
+        if ((mq >> lane) & 1) atomicAdd(b.coef + (int64_t)uq * 64 + lane, cz);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ms_bpush(GA g, BA b, KT* kt) {
+  kt_enter(kt);
+  k_ms_bpush_body(g, b);
+  kt_exit(kt);
+}
+
+// Finishes the level-d vertices whose arcs were split over several records
+// (delta = w(v) + the atomically added partial sums held in coef) or, on a
+// pushed level, every level-d vertex (delta = w(v) + sigma(v) x the pushed
+// sum of its successors' coef).
 __global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b) {
   Ctl* c = b.ctl;
   if (!c->fdone || c->bdone) return;
@@ -1336,19 +1419,24 @@ __global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b) {
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   const double mult = b.multbuf[lane];
+  const bool bpush = c->bpush;
+  const bool scaled = c->scaled;
+  const double inv_md = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   for (int64_t r = lo + gw; r < hi; r += tw) {
     if (b.rec_j[r] != 0) continue;
     int32_t v = b.rec_v[r];
     int64_t deg = g.off[v + 1] - g.off[v];
-    if (deg <= g.chunk) continue;
+    if (deg <= g.chunk && !bpush) continue;
     u64 L = b.rec_mask[r];
     const bool mine = (L >> lane) & 1;
     double contrib = 0.0;
     if (mine) {
       double* crow = b.coef + (int64_t)v * 64 + lane;
-      double dl = (g.w ? (double)g.w[v] : 0.0) + *crow;
       u64 sbits = b.sigma[(int64_t)v * 64 + lane];
-      double svr = c->scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
+      double svr = scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
+      double part = *crow;
+      if (bpush) part = part * (scaled ? svr * inv_md : svr);
+      double dl = (g.w ? (double)g.w[v] : 0.0) + part;
       *crow = (dl + 1.0) / svr;
       contrib = mult * dl;
       if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = dl;
@@ -1385,6 +1473,10 @@ __global__ void k_ms_cleanup(GA g, BA b) {
       if (r < src_hi) b.compmask[g.label[v]] = 0;
     }
   }
+  // the path-count rows of split vertices are zero outside a sweep
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.nsplit * 64;
+       t += (int64_t)gridDim.x * blockDim.x)
+    b.sigma[(int64_t)g.split_v[t >> 6] * 64 + (t & 63)] = 0;
 }
 
 // ====================================== workgroup-per-source sweeps
@@ -1858,12 +1950,12 @@ void DevGraph::release() {
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   if (toff != off) { fr(toff); fr(tadj); }
   if (off != foff) { fr(off); fr(adj); }
-  fr(foff); fr(fadj); fr(label); fr(comp_size); fr(comp_arcs); fr(all_v); fr(all_j);
+  fr(foff); fr(fadj); fr(label); fr(comp_size); fr(comp_arcs); fr(all_v); fr(all_j); fr(split_v);
   fr(round); fr(parent); fr(w); fr(root); fr(order); fr(d_n2o);
   d_n2o = nullptr; relabeled = false; h_n2o.clear(); h_o2n.clear();
   sigma_overflowed.store(false);
   off = foff = toff = nullptr; adj = fadj = tadj = nullptr; label = nullptr; comp_size = comp_arcs = nullptr;
-  all_v = all_j = nullptr; round = parent = root = order = nullptr; w = nullptr;
+  all_v = all_j = split_v = nullptr; nsplit = 0; round = parent = root = order = nullptr; w = nullptr;
   n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); h_sig.clear(); bytes = 0;
 }
 
@@ -2088,6 +2180,17 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
     HIP_TRY(hipStreamSynchronize(st));
   }
   {
+    std::vector<int32_t> sv;
+    for (size_t r = 0; r < rv.size(); r++)
+      if (rj[r] == 1) sv.push_back(rv[r]);
+    g.nsplit = (int64_t)sv.size();
+    TRY(dalloc(&g.split_v, std::max<int64_t>(g.nsplit, 1), &acc));
+    if (g.nsplit) {
+      HIP_TRY(hipMemcpyAsync(g.split_v, sv.data(), sizeof(int32_t) * g.nsplit, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+  }
+  {
     std::vector<int64_t> fo(off, off + n + 1);
     std::vector<int32_t> fv, fj;
     chunk_records(fo, prm.chunk, fv, fj);
@@ -2171,6 +2274,7 @@ struct Workspace {
   u64* stats = nullptr;   // 3 x 64
   u64* hstats = nullptr;  // pinned
   double* dbg_delta = nullptr;
+  int64_t* lvl_arcs = nullptr;  // [level] degree sum of the level's vertices
   double* lscale = nullptr;  // scaled path counts: per-level factors
   int64_t lscale_cap = 0;
   u64* lvmax = nullptr;
@@ -2218,7 +2322,7 @@ static GA make_ga(const DevGraph& g, const Params& prm) {
   GA a;
   a.off = g.off; a.adj = g.adj; a.toff = g.toff; a.tadj = g.tadj; a.directed = g.directed;
   a.label = g.label; a.comp_arcs = g.comp_arcs;
-  a.all_v = g.all_v; a.all_j = g.all_j; a.nall = g.nall;
+  a.all_v = g.all_v; a.all_j = g.all_j; a.nall = g.nall; a.split_v = g.split_v; a.nsplit = g.nsplit;
   a.w = g.heuristic ? (const uint64_t*)g.w : nullptr;
   a.chunk = g.chunk; a.n = g.n; a.m = g.m;
   a.flags = prm.kflags;
@@ -2237,7 +2341,8 @@ static BA make_ba(Workspace* ws, const Params& prm) {
   b.rec_v = ws->rec_v; b.rec_j = ws->rec_j; b.rec_mask = ws->rec_mask; b.cap = ws->cap; b.rec_lo = ws->rec_lo;
   b.compmask = ws->compmask; b.srcbuf = ws->srcbuf; b.multbuf = ws->multbuf; b.bc = ws->bc;
   b.st_n = ws->stats; b.st_m = ws->stats + 64; b.st_a = ws->stats + 128; b.dbg_delta = ws->dbg_delta;
-  b.alpha = prm.alpha_batched; b.beta = prm.beta_batched; b.gamma = prm.gamma_push; b.fuse = prm.fuse_bu; b.n = ws->n;
+  b.alpha = prm.alpha_batched; b.beta = prm.beta_batched; b.gamma = prm.gamma_push; b.fuse = prm.fuse_bu;
+  b.gamma_b = prm.gamma_back; b.lvl_arcs = ws->lvl_arcs; b.n = ws->n;
   b.lscale = ws->lscale; b.lscale_cap = ws->lscale_cap; b.lvmax = ws->lvmax;
   return b;
 }
@@ -2306,7 +2411,7 @@ void workspace_destroy(Workspace* ws) {
   if (ws->own_bc) fr(ws->bc);
   fr(ws->ctl); fr(ws->rec_v); fr(ws->rec_j); fr(ws->rec_mask); fr(ws->rec_lo); fr(ws->done);
   fr(ws->srcbuf); fr(ws->multbuf); fr(ws->depth); fr(ws->sigma1); fr(ws->delta1); fr(ws->coef1);
-  fr(ws->seen); fr(ws->lm); fr(ws->lb); fr(ws->lscale); fr(ws->lvmax); fr(ws->sigma); fr(ws->coef); fr(ws->compmask); fr(ws->stats); fr(ws->dbg_delta);
+  fr(ws->seen); fr(ws->lm); fr(ws->lb); fr(ws->lscale); fr(ws->lvmax); fr(ws->sigma); fr(ws->coef); fr(ws->compmask); fr(ws->stats); fr(ws->dbg_delta); fr(ws->lvl_arcs);
   fr(ws->cnt0); fr(ws->cntS); fr(ws->sa); fr(ws->srcK); fr(ws->srcdev);
   if (ws->hctl) (void)hipHostFree(ws->hctl);
   if (ws->hkt) (void)hipHostFree(ws->hkt);
@@ -2377,6 +2482,7 @@ static int ws_init(Workspace* ws, const DevGraph& g, int mode, int width, const 
   } else {
     TRY(dalloc(&ws->seen, n, acc));
     TRY(dalloc(&ws->lm, 3 * n, acc));
+    TRY(dalloc(&ws->lvl_arcs, n + 3, acc));
     // a sweep has at most 2 x (hub BFS depth) + 1 levels within the hubs'
     // component; deeper components elsewhere grow the table on demand
     ws->lscale_cap = std::min<int64_t>(n + 2, std::max<int64_t>(4096, 2 * (int64_t)g.depth_hint + 8));
@@ -2393,6 +2499,7 @@ static int ws_init(Workspace* ws, const DevGraph& g, int mode, int width, const 
     hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->seen, 0ull, n);
     hipLaunchKernelGGL(k_fill_u64, grid_for(3 * n), 256, 0, st, ws->lm, 0ull, 3 * n);
     hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->compmask, 0ull, n);
+    hipLaunchKernelGGL(k_fill_u64, grid_for(64 * n), 256, 0, st, ws->sigma, 0ull, 64 * n);
   }
   TRY(alloc_records(ws, record_capacity(g, mode)));
   HIP_TRY(hipStreamSynchronize(st));
@@ -2487,9 +2594,13 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
           ktacc(1, Gk);
           hipLaunchKernelGGL(k_ms_push, Gk, B, 0, st, ga, ba);
         } else {
-          hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, ba.ctl);
+          hipLaunchKernelGGL(k_ms_bdecide, 1, 1, 0, st, ga, ba);
           hipLaunchKernelGGL(k_ms_back, Gk, B, 0, st, ga, ba, tk(2));
           ktacc(2, Gk);
+          if (prm.gamma_back > 0) {
+            hipLaunchKernelGGL(k_ms_bpush, Gk, B, 0, st, ga, ba, tk(2));
+            ktacc(2, Gk);
+          }
           hipLaunchKernelGGL(k_ms_bfin, Gk, B, 0, st, ga, ba);
         }
       }
@@ -2512,7 +2623,7 @@ static bool same_launch(const Params& a, const Params& b) {
   return a.block == b.block && a.grid == b.grid && a.grid_plain == b.grid_plain && a.chunk == b.chunk &&
          a.alpha_plain == b.alpha_plain &&
          a.alpha_batched == b.alpha_batched && a.beta_plain == b.beta_plain && a.beta_batched == b.beta_batched &&
-         a.gamma_push == b.gamma_push && a.fuse_bu == b.fuse_bu &&
+         a.gamma_push == b.gamma_push && a.gamma_back == b.gamma_back && a.fuse_bu == b.fuse_bu &&
          a.steps == b.steps && a.ktime == b.ktime &&
          a.kflags == b.kflags && a.sgrid == b.sgrid;
 }
@@ -2578,6 +2689,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
         hipLaunchKernelGGL(k_fill_u64, grid_for(3 * n), 256, 0, st, ws->lm, 0ull, 3 * n);
         hipLaunchKernelGGL(k_fill_i32, grid_for(3 * ws->nw), 256, 0, st, (int32_t*)ws->lb, 0, 3 * ws->nw);
         hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->compmask, 0ull, n);
+        hipLaunchKernelGGL(k_fill_u64, grid_for(64 * n), 256, 0, st, ws->sigma, 0ull, 64 * n);
       }
       hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, ws->done, 0, n);
       HIP_TRY(hipStreamSynchronize(st));
@@ -2620,7 +2732,8 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       hipLaunchKernelGGL(k_plain_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, st, pa);
     } else {
       BA ba = make_ba(ws, prm);
-      hipLaunchKernelGGL(k_ms_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, st, ga, ba);
+      hipLaunchKernelGGL(k_ms_cleanup, grid_for(std::max(std::min<int64_t>(ws->hctl->rec_count, ws->cap), 64 * g.nsplit)),
+                         256, 0, st, ga, ba);
     }
   }
   return BCMI_OK;
@@ -2633,7 +2746,8 @@ static int sweep_cleanup(const DevGraph& g, Workspace* ws, const Params& prm) {
     hipLaunchKernelGGL(k_plain_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, ws->st, pa);
   } else {
     BA ba = make_ba(ws, prm);
-    hipLaunchKernelGGL(k_ms_cleanup, grid_for(std::min<int64_t>(ws->hctl->rec_count, ws->cap)), 256, 0, ws->st, ga, ba);
+    hipLaunchKernelGGL(k_ms_cleanup, grid_for(std::max(std::min<int64_t>(ws->hctl->rec_count, ws->cap), 64 * g.nsplit)),
+                       256, 0, ws->st, ga, ba);
   }
   HIP_TRY(hipStreamSynchronize(ws->st));
   return BCMI_OK;

@@ -123,7 +123,11 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * work item), "alpha_plain"/"alpha_batched" (x1000: top-down -> bottom-up
  * switch when frontier arcs > alpha * unvisited arcs), "beta_plain"/
  * "beta_batched" (and only when frontier arcs >= all arcs / beta),
- * "fuse_bu" (1: bottom-up levels also sum path counts in their scan),
+ * "fuse_bu" (bottom-up levels also sum path counts in their scan: 0 never,
+ * 1 for vertices held in one work record, 2 for all - the default),
+ * "gamma_back" (batched backward levels whose successor level holds
+ * fewer than 1/gamma_back of their arcs take their sums pushed from the
+ * successors; 0, the default, always pulls),
  * "gamma_push" (batched top-down levels whose frontier holds fewer than
  * all arcs / gamma push path counts from the frontier; 0 always pulls),
  * "steps" (levels per
