@@ -232,6 +232,10 @@ def main():
     else:
         kbytes = union_part / 2.0
     achieved = kbytes / (k_ms / 1e3) / 1e9 if k_ms > 0 and kt else None
+    # with pushed backward levels the backward timer also covers k_ms_bpush and k_ms_bfin
+    kname = f"k_ms_{args.kernel}"
+    if args.kernel == "back" and e.get_param("gamma_back") > 0:
+        kname = "k_ms_back+k_ms_bpush+k_ms_bfin"
     log(f"kernel {args.kernel}: {k_launch} launches, {k_ms:.1f} ms total, "
         f"avg {k_ms / max(k_launch, 1) * 1e3:.1f} us, achieved {achieved} GB/s")
 
@@ -299,7 +303,8 @@ def main():
         "config": {"workload": f"batched Brandes BC, {gdesc}, {len(timed)} sampled sources in batches of {width}",
                    "n": g.n, "arcs": g.m, "batch": width,
                    "tuned": {k: v for k, v in (tuned or {}).items() if k in
-                             ("chunk", "block", "grid", "alpha_batched", "width")}},
+                             ("chunk", "block", "grid", "alpha_batched", "beta_batched", "gamma_push",
+                              "gamma_back", "steps", "width")}},
         "plain_gteps": round(gteps_plain, 3) if gteps_plain else None,
         "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain else None,
         "model_bytes_gbs": round(st["bytes"] / t_batched / 1e9, 1),
@@ -307,7 +312,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic, "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE)",
-                     "kernel": f"k_ms_{args.kernel}" if kt else None,
+                     "kernel": kname if kt else None,
                      "avg_launch_us": round(k_ms / max(k_launch, 1) * 1e3, 2)},
         "cpu_baseline": cpu,
         "serving": serving,

@@ -1410,7 +1410,7 @@ __global__ void __launch_bounds__(256) k_ms_bpush(GA g, BA b, KT* kt) {
 // (delta = w(v) + the atomically added partial sums held in coef) or, on a
 // pushed level, every level-d vertex (delta = w(v) + sigma(v) x the pushed
 // sum of its successors' coef).
-__global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b) {
+__device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (!c->fdone || c->bdone) return;
   const int d = c->blevel;
@@ -1444,6 +1444,12 @@ __global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b) {
     contrib = wave_sum_d(contrib);
     if (lane == 0) b.bc[v] += contrib;
   }
+}
+
+__global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b, KT* kt) {
+  kt_enter(kt);
+  k_ms_bfin_body(g, b);
+  kt_exit(kt);
 }
 
 // Resets the batch state for the next sweep.  seen is cleared for every
@@ -2597,11 +2603,16 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
           hipLaunchKernelGGL(k_ms_bdecide, 1, 1, 0, st, ga, ba);
           hipLaunchKernelGGL(k_ms_back, Gk, B, 0, st, ga, ba, tk(2));
           ktacc(2, Gk);
+          // With pushed levels the backward work is shared by three kernels,
+          // so the backward timer covers all of them.
           if (prm.gamma_back > 0) {
             hipLaunchKernelGGL(k_ms_bpush, Gk, B, 0, st, ga, ba, tk(2));
             ktacc(2, Gk);
+            hipLaunchKernelGGL(k_ms_bfin, Gk, B, 0, st, ga, ba, tk(2));
+            ktacc(2, Gk);
+          } else {
+            hipLaunchKernelGGL(k_ms_bfin, Gk, B, 0, st, ga, ba, (KT*)nullptr);
           }
-          hipLaunchKernelGGL(k_ms_bfin, Gk, B, 0, st, ga, ba);
         }
       }
     }
