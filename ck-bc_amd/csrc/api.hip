@@ -118,6 +118,8 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "fuse_bu") {
     if (v < 0 || v > 2) return set_error(BCMI_ERR_ARG, "fuse_bu must be 0, 1 or 2");
     p.fuse_bu = (int)v;
+  } else if (k == "narrow_sigma") {
+    p.narrow_sigma = v ? 1 : 0;
   } else if (k == "gamma_back") {
     if (v < 0) return set_error(BCMI_ERR_ARG, "gamma_back must be >= 0");
     p.gamma_back = v;
@@ -192,6 +194,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "beta_batched") *v = p.beta_batched;
   else if (k == "gamma_push") *v = p.gamma_push;
   else if (k == "gamma_back") *v = p.gamma_back;
+  else if (k == "narrow_sigma") *v = p.narrow_sigma;
   else if (k == "fuse_bu") *v = p.fuse_bu;
   else if (k == "steps") *v = p.steps;
   else if (k == "width") *v = p.width;

@@ -34,6 +34,8 @@ struct Params {
                                 // unsplit vertices, 2 of all (split vertices add per record atomically)
   int64_t gamma_push = 8;       // batched top-down levels with frontier arcs < all arcs / gamma push
                                 // path counts from the frontier (0: always pull from parents)
+  int narrow_sigma = 1;         // exact batched sweeps first keep path counts in 32 bits (redone
+                                // with 64 bits when a count passes 2^32-1)
   int64_t gamma_back = 0;       // batched backward levels whose successor level holds fewer than
                                 // 1/gamma_back of their arcs push coef from it (0: always pull)
   int steps = 16;           // levels per captured graph

@@ -29,7 +29,7 @@ struct Ctl {
   int32_t scaled;     // batched: path counts held as FP64 scaled per level
   int32_t push;       // batched: this level's path counts are pushed from the frontier (k_ms_push)
   int32_t bpush;      // batched: this backward level pulls nothing; the successor level pushes (k_ms_bpush)
-  int32_t pad0;
+  int32_t narrow;     // batched: path counts held in 32-bit rows
   int64_t rec_count;  // records appended so far
   int64_t arcs_next;  // degree sum of vertices appended during this step
   int64_t verts_next;
@@ -96,6 +96,7 @@ struct DevGraph {
   // set once a batched sweep's exact path counts wrapped (auto mode then
   // starts later batches in scaled FP64)
   mutable std::atomic<bool> sigma_overflowed{false};
+  mutable std::atomic<bool> narrow_overflowed{false};  // a batch's counts passed 2^32-1
   size_t bytes = 0;
   void release();
 };

@@ -260,6 +260,44 @@ struct BA {
   u64* lvmax;          // [64] running max of the level being built (double bits)
 };
 
+// Path-count rows.  Exact sweeps first keep the counts of a batch in 32-bit
+// rows ([v][64] of uint32, 256 bytes) - half the bytes of every gather; a
+// batch whose counts pass 2^32-1 is redone with 64-bit rows (exact integers
+// or FP64 bits when scaled).  The kernels are built for both widths (`nar` is
+// a template constant in their loops).
+__device__ __forceinline__ u64 sg_ld(const u64* sg, bool nar, int64_t i) {
+  return nar ? (u64)reinterpret_cast<const uint32_t*>(sg)[i] : sg[i];
+}
+__device__ __forceinline__ void sg_st(u64* sg, bool nar, int64_t i, u64 x) {
+  if (nar) reinterpret_cast<uint32_t*>(sg)[i] = (uint32_t)x;
+  else sg[i] = x;
+}
+// One gather of an unrolled group.  32-bit rows load unconditionally (an
+// unused lane reads element 0) so the group's loads stay in flight together
+// instead of each waiting behind a branch.
+template <bool NAR>
+__device__ __forceinline__ u64 sg_gather(const u64* sg, bool use, int64_t i) {
+  if (NAR) {
+    const uint32_t x = reinterpret_cast<const uint32_t*>(sg)[use ? i : 0];
+    return use ? (u64)x : 0;
+  }
+  return use ? sg[i] : 0;
+}
+// stores an exact count; true when it does not fit the row
+__device__ __forceinline__ bool sg_put(u64* sg, bool nar, int64_t i, u64 x) {
+  sg_st(sg, nar, i, x);
+  return nar && x > 0xffffffffull;
+}
+// adds an exact count atomically; true when the row's width overflows
+__device__ __forceinline__ bool sg_add(u64* sg, bool nar, int64_t i, u64 x) {
+  if (nar) {
+    const uint32_t o = atomicAdd(reinterpret_cast<uint32_t*>(sg) + i, (uint32_t)x);
+    return x > 0xffffffffull || o + (uint32_t)x < o;
+  }
+  const u64 o = atomicAdd(sg + i, x);
+  return o + x < o;
+}
+
 // Work-list appends are staged per wave in LDS and flushed with one global
 // atomic per WBUF records, so the shared record counter is not hammered by
 // every wave iteration.  The frontier degree and vertex counts are kept in
@@ -740,7 +778,7 @@ __global__ void k_plain_cleanup(PA p) {
 // seen (sources that reached v so far) and three rotating level arrays lm[k]
 // with lm[d%3] = mask of sources for which v has depth d, kept exact for the
 // levels a kernel reads.
-__global__ void k_ms_init(GA g, BA b, int k, int scaled) {
+__global__ void k_ms_init(GA g, BA b, int k, int scaled, int narrow) {
   Ctl* c = b.ctl;
   const int i = threadIdx.x;  // one block of 64 threads
   u64* lm0 = b.lm;
@@ -753,6 +791,7 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled) {
     c->arcs_seen = 0; c->sum_a = 0; c->n_td = 0; c->n_bu = 0;
     c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 0; c->uni_arcs = 0;
     c->sigma_ovf = 0; c->width = k; c->scaled = scaled; c->push = 0; c->bpush = 0;
+    c->narrow = narrow && !scaled;
     b.rec_lo[0] = 0;
   }
   b.st_n[i] = 0; b.st_m[i] = 0; b.st_a[i] = 0;
@@ -766,7 +805,7 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled) {
     if (g.flags & KF_GATE) lb_set(b.lb, s);
     atomicOr(&b.seen[s], 1ull << i);
     atomicOr(&b.compmask[g.label[s]], 1ull << i);
-    b.sigma[(int64_t)s * 64 + i] = scaled ? (u64)__double_as_longlong(1.0) : 1ull;
+    sg_st(b.sigma, narrow && !scaled, (int64_t)s * 64 + i, scaled ? (u64)__double_as_longlong(1.0) : 1ull);
   }
   __syncthreads();
   if (i < k) {
@@ -903,6 +942,7 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 // the records of a split vertex (fuse_bu 2) add theirs atomically into its
 // row, whose columns are zero until the source reaches it.
 #define UNR_BU 8
+template <bool NAR>
 __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone || c->mode != 1) return;
@@ -916,6 +956,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   const bool fuse = b.fuse;
   const bool scaled = c->scaled;
+  constexpr bool nar = NAR;
   const double inv_m = (fuse && scaled) ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   u64 ca = 0;
   int ovf = 0;
@@ -961,7 +1002,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
           const int32_t uq = rl32(u, q[t]);
           const u64 nq = rl64(need, q[t]);
           const bool use = okq[t] && ((nq >> lane) & 1);
-          x[t] = use ? b.sigma[(int64_t)uq * 64 + lane] : 0;
+          x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
           ca += use;
         }
         if (scaled) {
@@ -1003,10 +1044,9 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
           mx = fmax(mx, o + raw);
         }
       } else if (!split) {
-        *row = sacc;
+        if (sg_put(b.sigma, nar, (int64_t)v * 64 + lane, sacc)) ovf = 1;
       } else {
-        const u64 o = atomicAdd(row, sacc);
-        if (o + sacc < o) ovf = 1;
+        if (sg_add(b.sigma, nar, (int64_t)v * 64 + lane, sacc)) ovf = 1;
       }
     }
     app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
@@ -1021,7 +1061,8 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
 
 __global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  k_ms_bu_body(g, b);
+  if (b.ctl->narrow) k_ms_bu_body<true>(g, b);
+  else k_ms_bu_body<false>(g, b);
   kt_exit(kt);
 }
 
@@ -1029,6 +1070,7 @@ __global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
 // neighbours u that are at depth d for source i (the parents).  The first
 // record of v publishes its mask, marks it seen and counts statistics.  Also
 // clears level d-1 from its rotating array (not read at this step).
+template <bool NAR>
 __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone) return;
@@ -1047,6 +1089,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   const bool scaled = c->scaled;
   const bool push = c->push;  // k_ms_push sums the parents
   const bool fused = c->mode == 1 && b.fuse;  // k_ms_bu summed the parents (of unsplit vertices if fuse 1)
+  constexpr bool nar = NAR;
   const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   double mx = 0.0;
   for (int64_t r = lo + gw; r < hi; r += tw) {
@@ -1096,7 +1139,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
           int32_t uq = rl32(u, q[t]);
           u64 fq = rl64(F, q[t]);
           bool use = ok[t] && ((fq >> lane) & 1);
-          x[t] = use ? b.sigma[(int64_t)uq * 64 + lane] : 0;
+          x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
           ca += use;
         }
         if (scaled) {
@@ -1116,7 +1159,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
     if (done_by_bu) {
       // row written by k_ms_bu
     } else if (push) {
-      if (mine && j == 0) *row = 0ull;  // k_ms_push adds the parents' counts next
+      if (mine && j == 0) sg_st(b.sigma, nar, (int64_t)v * 64 + lane, 0);  // k_ms_push adds the parents' counts next
     } else if (scaled) {
       double raw = accf * inv_m;
       if (mine) {
@@ -1125,10 +1168,9 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
         mx = fmax(mx, raw);
       }
     } else if (nch == 1) {
-      if (mine) *row = acc;
+      if (mine && sg_put(b.sigma, nar, (int64_t)v * 64 + lane, acc)) ovf = 1;
     } else if (mine) {
-      u64 o = atomicAdd(row, acc);
-      if (o + acc < o) ovf = 1;
+      if (sg_add(b.sigma, nar, (int64_t)v * 64 + lane, acc)) ovf = 1;
     }
     if (j == 0) {
       u64 so = b.seen[v];
@@ -1168,7 +1210,8 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
 
 __global__ void __launch_bounds__(256) k_ms_sigma(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  k_ms_sigma_body(g, b);
+  if (b.ctl->narrow) k_ms_sigma_body<true>(g, b);
+  else k_ms_sigma_body<false>(g, b);
   kt_exit(kt);
 }
 
@@ -1179,6 +1222,7 @@ __global__ void __launch_bounds__(256) k_ms_sigma(GA g, BA b, KT* kt) {
 // level's, which the pull in k_ms_sigma would scan.  Exact counts use
 // returning atomics to detect a wrap; scaled counts add raw values and keep
 // the level maximum (every returned sum is a lower bound of a final one).
+template <bool NAR>
 __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone || !c->push) return;
@@ -1189,6 +1233,7 @@ __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   const bool scaled = c->scaled;
+  constexpr bool nar = NAR;
   const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   u64 ca = 0;
   int ovf = 0;
@@ -1199,7 +1244,7 @@ __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
     const u64 F = b.rec_mask[r - j];
     const int64_t e0 = g.off[u] + (int64_t)j * g.chunk;
     const int64_t e1 = min(g.off[u + 1], e0 + g.chunk);
-    const u64 sb = ((F >> lane) & 1) ? b.sigma[(int64_t)u * 64 + lane] : 0ull;
+    const u64 sb = ((F >> lane) & 1) ? sg_ld(b.sigma, nar, (int64_t)u * 64 + lane) : 0ull;
     const double sf = scaled ? __longlong_as_double((long long)sb) * inv_m : 0.0;
     for (int64_t eb = e0; eb < e1; eb += 64) {
       const int64_t e = eb + lane;
@@ -1221,9 +1266,8 @@ __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
           if (scaled) {
             double o = atomicAdd((double*)row, sf);
             mx = fmax(mx, o + sf);
-          } else {
-            u64 o = atomicAdd(row, sb);
-            if (o + sb < o) ovf = 1;
+          } else if (sg_add(b.sigma, nar, (int64_t)vq * 64 + lane, sb)) {
+            ovf = 1;
           }
         }
       }
@@ -1234,7 +1278,10 @@ __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
   if (ovf) c->sigma_ovf = 1;
 }
 
-__global__ void __launch_bounds__(256) k_ms_push(GA g, BA b) { k_ms_push_body(g, b); }
+__global__ void __launch_bounds__(256) k_ms_push(GA g, BA b) {
+  if (b.ctl->narrow) k_ms_push_body<true>(g, b);
+  else k_ms_push_body<false>(g, b);
+}
 
 // Backward level d for a batch: lane i accumulates, in adjacency order,
 // coef[z][i] * sigma[v][i] over the successors z of v for source i
@@ -1242,6 +1289,7 @@ __global__ void __launch_bounds__(256) k_ms_push(GA g, BA b) { k_ms_push_body(g,
 // coef[v][i] = (delta+1)/sigma and bc[v] += sum_i mult_i * delta_i.  Also
 // publishes level d into lm[d%3] for the next (shallower) step and clears
 // level d+2 from lm[(d+2)%3].
+template <bool NAR>
 __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (!c->fdone || c->bdone) return;
@@ -1282,7 +1330,7 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
     int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
     const bool mine = (L >> lane) & 1;
     const double wv = g.w ? (double)g.w[v] : 0.0;
-    const u64 sbits = mine ? b.sigma[(int64_t)v * 64 + lane] : 1ull;
+    const u64 sbits = mine ? sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane) : 1ull;
     const double svr = !mine ? 1.0 : scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
     const double sv = scaled ? svr * inv_md : svr;
     double acc = (nch == 1) ? wv : 0.0;
@@ -1354,7 +1402,8 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
 
 __global__ void __launch_bounds__(256) k_ms_back(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  k_ms_back_body(g, b);
+  if (b.ctl->narrow) k_ms_back_body<true>(g, b);
+  else k_ms_back_body<false>(g, b);
   kt_exit(kt);
 }
 
@@ -1410,6 +1459,7 @@ __global__ void __launch_bounds__(256) k_ms_bpush(GA g, BA b, KT* kt) {
 // (delta = w(v) + the atomically added partial sums held in coef) or, on a
 // pushed level, every level-d vertex (delta = w(v) + sigma(v) x the pushed
 // sum of its successors' coef).
+template <bool NAR>
 __device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (!c->fdone || c->bdone) return;
@@ -1432,7 +1482,7 @@ __device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
     double contrib = 0.0;
     if (mine) {
       double* crow = b.coef + (int64_t)v * 64 + lane;
-      u64 sbits = b.sigma[(int64_t)v * 64 + lane];
+      u64 sbits = sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane);
       double svr = scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
       double part = *crow;
       if (bpush) part = part * (scaled ? svr * inv_md : svr);
@@ -1448,7 +1498,8 @@ __device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
 
 __global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  k_ms_bfin_body(g, b);
+  if (b.ctl->narrow) k_ms_bfin_body<true>(g, b);
+  else k_ms_bfin_body<false>(g, b);
   kt_exit(kt);
 }
 
@@ -1482,7 +1533,7 @@ __global__ void k_ms_cleanup(GA g, BA b) {
   // the path-count rows of split vertices are zero outside a sweep
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.nsplit * 64;
        t += (int64_t)gridDim.x * blockDim.x)
-    b.sigma[(int64_t)g.split_v[t >> 6] * 64 + (t & 63)] = 0;
+    sg_st(b.sigma, c->narrow, (int64_t)g.split_v[t >> 6] * 64 + (t & 63), 0);
 }
 
 // ====================================== workgroup-per-source sweeps
@@ -1960,6 +2011,7 @@ void DevGraph::release() {
   fr(round); fr(parent); fr(w); fr(root); fr(order); fr(d_n2o);
   d_n2o = nullptr; relabeled = false; h_n2o.clear(); h_o2n.clear();
   sigma_overflowed.store(false);
+  narrow_overflowed.store(false);
   off = foff = toff = nullptr; adj = fadj = tadj = nullptr; label = nullptr; comp_size = comp_arcs = nullptr;
   all_v = all_j = split_v = nullptr; nsplit = 0; round = parent = root = order = nullptr; w = nullptr;
   n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); h_sig.clear(); bytes = 0;
@@ -2305,6 +2357,7 @@ struct Workspace {
   std::vector<Workspace*> helpers;
   const DevGraph* helpers_graph = nullptr;
   // workgroup-per-source sweeps (deep graphs): per-slot state and per-run buffers
+  int sig_layout = -1;  // layout of the last batched sweep's path-count rows: 1 narrow, 0 wide, -1 all zero
   int64_t wg_slots = 0, wg_lcap = 0;
   int64_t wg_req = 0;                 // slots asked for (wg_slots may be fewer: device memory)
   int32_t* wg_depth = nullptr;
@@ -2642,7 +2695,8 @@ static bool same_launch(const Params& a, const Params& b) {
 // Runs one sweep (plain: srcbuf[0]; batched: srcbuf[0..k)) whose inputs are
 // already in the workspace buffers.  keep != 0 skips the cleanup (debug
 // readback), the caller then calls sweep_cleanup.
-static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, bool keep, int* levels, int scaled = 0) {
+static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, bool keep, int* levels, int scaled = 0,
+                 int narrow = 0) {
   if (!ws->gfwd || ws->cap_graph != &g || !same_launch(ws->cap_prm, prm)) TRY(capture(g, ws, prm));
   GA ga = make_ga(g, prm);
   hipStream_t st = ws->st;
@@ -2651,8 +2705,14 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       PA pa = make_pa(ws, prm);
       hipLaunchKernelGGL(k_plain_init, 1, 1, 0, st, ga, pa);
     } else {
+      // Split vertices' rows are zero between sweeps in the layout in use;
+      // a change of layout starts from an all-zero array.
+      const int lay = narrow && !scaled ? 1 : 0;
+      if (ws->sig_layout >= 0 && ws->sig_layout != lay)
+        hipLaunchKernelGGL(k_fill_u64, grid_for(64 * ws->n), 256, 0, st, ws->sigma, 0ull, 64 * ws->n);
+      ws->sig_layout = lay;
       BA ba = make_ba(ws, prm);
-      hipLaunchKernelGGL(k_ms_init, 1, 64, 0, st, ga, ba, k, scaled);
+      hipLaunchKernelGGL(k_ms_init, 1, 64, 0, st, ga, ba, k, scaled, narrow);
     }
     if (prm.trace) {
       // Diagnostic: one graph launch at a time, level statistics on stderr.
@@ -2701,6 +2761,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
         hipLaunchKernelGGL(k_fill_i32, grid_for(3 * ws->nw), 256, 0, st, (int32_t*)ws->lb, 0, 3 * ws->nw);
         hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, ws->compmask, 0ull, n);
         hipLaunchKernelGGL(k_fill_u64, grid_for(64 * n), 256, 0, st, ws->sigma, 0ull, 64 * n);
+        ws->sig_layout = -1;
       }
       hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, ws->done, 0, n);
       HIP_TRY(hipStreamSynchronize(st));
@@ -2862,7 +2923,14 @@ static int run_batch(const DevGraph& g, Workspace* w, const Params& prm, const i
   int lv = 0;
   // auto mode: once a batch of this graph has overflowed, later ones start scaled
   int scaled = prm.sigma == 1 || (prm.sigma == 2 && g.sigma_overflowed.load());
-  TRY(sweep(g, w, prm, k, false, &lv, scaled));
+  const int narrow = !scaled && prm.narrow_sigma && !g.narrow_overflowed.load();
+  TRY(sweep(g, w, prm, k, false, &lv, scaled, narrow));
+  if (narrow && w->hctl->sigma_ovf) {
+    // a count passed 2^32-1 (the backward pass was skipped): redo the batch
+    // with 64-bit rows, as later batches of this graph will start
+    g.narrow_overflowed.store(true);
+    TRY(sweep(g, w, prm, k, false, &lv, 0, 0));
+  }
   if (w->hctl->sigma_ovf && prm.sigma == 2) {
     // exact counts wrapped: the backward pass was skipped, redo the batch scaled
     g.sigma_overflowed.store(true);
@@ -3278,8 +3346,15 @@ int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32
   HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, 64 * sizeof(int32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, 64 * sizeof(double), hipMemcpyHostToDevice, st));
   int lv;
-  int rc = sweep(g, ws, prm, k, true, &lv);
+  const int narrow = prm.narrow_sigma && !g.narrow_overflowed.load();
+  int rc = sweep(g, ws, prm, k, true, &lv, 0, narrow);
+  if (rc == BCMI_OK && narrow && ws->hctl->sigma_ovf) {
+    // counts past 2^32-1: redo with 64-bit rows
+    rc = sweep_cleanup(g, ws, prm);
+    if (rc == BCMI_OK) rc = sweep(g, ws, prm, k, true, &lv, 0, 0);
+  }
   if (rc != BCMI_OK) return rc;
+  const bool nar = ws->hctl->narrow;
   int64_t nrec = std::min<int64_t>(ws->hctl->rec_count, ws->cap);
   int last = ws->hctl->last;
   std::vector<int32_t> rv(nrec), rj(nrec);
@@ -3294,6 +3369,14 @@ int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32
   HIP_TRY(hipMemcpyAsync(sg.data(), ws->sigma, sizeof(u64) * 64 * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(dd.data(), ws->dbg_delta, sizeof(double) * 64 * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  if (nar) {  // 32-bit rows: widen in place, from the end
+    const char* s32 = reinterpret_cast<const char*>(sg.data());
+    for (size_t i = (size_t)64 * n; i-- > 0;) {
+      uint32_t x;
+      memcpy(&x, s32 + 4 * i, 4);
+      sg[i] = x;
+    }
+  }
   for (int q = 0; q < k; q++)
     for (int64_t v = 0; v < n; v++) { depth[(int64_t)q * n + v] = -1; sigma[(int64_t)q * n + v] = 0; delta[(int64_t)q * n + v] = 0.0; }
   for (int d = 0; d <= last; d++)

@@ -320,3 +320,30 @@ def test_batched_level_variants(gamma_back, fuse_bu, chunk, sigma):
                 e.run(src, mode=bcmi.MODE_BATCHED)
                 assert bcmi.rel_err(e.bc(), ref) <= BC_TOL
             e.close()
+
+
+@pytest.mark.parametrize("narrow", [1, 0])
+def test_batched_counts_past_32_bits(narrow):
+    """Hypercube Q13: sigma(s -> v) = popcount(s ^ v)!, and 13! > 2^32 - 1.
+    With 32-bit rows first (narrow_sigma 1) the batch is redone with 64-bit
+    rows: depths and path counts still match the closed form exactly, BC
+    matches the oracle, and a later run on the same graph agrees."""
+    from math import factorial
+    d = 13
+    g = bcmi.Graph.from_edges(*G.hypercube(d))
+    src = np.array([0, 5, 1234, 8191, 4096, 77], np.int32)
+    e = bcmi.Engine(narrow_sigma=narrow)
+    e.upload(g)
+    dm, sgm, _ = e.msbfs(src)
+    v = np.arange(g.n)
+    fact = np.array([factorial(k) for k in range(d + 1)], np.uint64)
+    for q, s in enumerate(src):
+        pc = np.array([bin(x).count("1") for x in (v ^ s)])
+        assert np.array_equal(dm[q], pc)
+        assert np.array_equal(sgm[q], fact[pc])
+    ref = O.brandes(g.off, g.adj, src, nthreads=8)
+    for _ in range(2):
+        e.reset()
+        e.run(src, mode=bcmi.MODE_BATCHED)
+        assert bcmi.rel_err(e.bc(), ref) <= BC_TOL
+    e.close()
