@@ -27,10 +27,12 @@ import bcmi  # noqa: E402
 SPACE = {
     "block": [64, 128, 256],
     "grid": [1024, 2048, 4096, 8192, 16384],
+    "grid_back": [0, 4096, 8192, 16384],
     "alpha_batched": [250, 500, 1000, 2000, 4000, 1 << 60],
     "beta_batched": [4, 8, 16, 32, 64],
     "gamma_push": [0, 4, 8, 16, 32],
     "fuse_bu": [1, 2],
+    "pack_bu": [0, 1],
     "gamma_back": [0, 4, 8, 16],
     "steps": [8, 10, 12, 16],
     "width": [32, 48, 64],
@@ -59,15 +61,15 @@ def measure(e, src, mode, reps=1):
 
 def tune(g, src_batched, src_plain, log=print):
     params = {"chunk": 256, "block": 256, "grid": 4096, "grid_plain": 4096, "alpha_batched": 1000, "beta_batched": 16, "gamma_push": 8,
-              "fuse_bu": 2, "gamma_back": 0, "steps": 16, "width": 64,
+              "fuse_bu": 2, "pack_bu": 1, "gamma_back": 0, "steps": 16, "width": 64, "grid_back": 0,
               "alpha_plain": 1000}
     best_overall = (-1.0, None)
     for ch in CHUNKS:
         e = bcmi.Engine(0, chunk=ch)
         e.upload(g)
         cur = dict(params, chunk=ch)
-        for k in ("block", "grid", "alpha_batched", "beta_batched", "gamma_push", "fuse_bu", "gamma_back", "steps",
-                  "width"):
+        for k in ("block", "grid", "grid_back", "alpha_batched", "beta_batched", "gamma_push", "fuse_bu", "pack_bu",
+                  "gamma_back", "steps", "width"):
             if k != "chunk":
                 e.set_param(k, cur[k])
         e.run(src_batched[:64])  # capture
@@ -121,8 +123,8 @@ def apply(engine, table, key):
     p = table.get(key) or table.get("default")
     if not p:
         return None
-    for k in ("chunk", "block", "grid", "grid_plain", "alpha_batched", "beta_batched", "gamma_push", "fuse_bu",
-              "gamma_back", "steps", "alpha_plain", "width"):
+    for k in ("chunk", "block", "grid", "grid_back", "grid_plain", "alpha_batched", "beta_batched", "gamma_push",
+              "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width"):
         if k in p:
             engine.set_param(k, int(p[k]))
     return p

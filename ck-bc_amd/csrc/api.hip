@@ -102,6 +102,9 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "grid") {
     if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "grid must be in [1,65536]");
     p.grid = (int)v;
+  } else if (k == "grid_back") {
+    if (v < 0 || v > 65536) return set_error(BCMI_ERR_ARG, "grid_back must be in [0,65536]");
+    p.grid_back = (int)v;
   } else if (k == "grid_plain") {
     if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "grid_plain must be in [1,65536]");
     p.grid_plain = (int)v;
@@ -118,6 +121,8 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
   } else if (k == "fuse_bu") {
     if (v < 0 || v > 2) return set_error(BCMI_ERR_ARG, "fuse_bu must be 0, 1 or 2");
     p.fuse_bu = (int)v;
+  } else if (k == "pack_bu") {
+    p.pack_bu = v ? 1 : 0;
   } else if (k == "narrow_sigma") {
     p.narrow_sigma = v ? 1 : 0;
   } else if (k == "gamma_back") {
@@ -187,6 +192,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   if (k == "block") *v = p.block;
   else if (k == "grid") *v = p.grid;
   else if (k == "grid_plain") *v = p.grid_plain;
+  else if (k == "grid_back") *v = p.grid_back;
   else if (k == "chunk") *v = p.chunk;
   else if (k == "alpha_plain") *v = p.alpha_plain;
   else if (k == "alpha_batched") *v = p.alpha_batched;
@@ -195,6 +201,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "gamma_push") *v = p.gamma_push;
   else if (k == "gamma_back") *v = p.gamma_back;
   else if (k == "narrow_sigma") *v = p.narrow_sigma;
+  else if (k == "pack_bu") *v = p.pack_bu;
   else if (k == "fuse_bu") *v = p.fuse_bu;
   else if (k == "steps") *v = p.steps;
   else if (k == "width") *v = p.width;

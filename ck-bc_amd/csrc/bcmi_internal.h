@@ -25,6 +25,7 @@ struct Params {
   int block = 256;          // threads per block (multiple of 64)
   int grid = 4096;          // blocks per batched expansion kernel (about twice what is resident)
   int grid_plain = 4096;    // blocks per plain-sweep kernel
+  int grid_back = 0;        // blocks per batched backward kernel (0: grid)
   int chunk = 256;          // max edges per work item
   int64_t alpha_plain = 1000;   // x1000; bottom-up when frontier arcs > alpha * unvisited arcs
   int64_t alpha_batched = 1000; // x1000
@@ -34,6 +35,7 @@ struct Params {
                                 // unsplit vertices, 2 of all (split vertices add per record atomically)
   int64_t gamma_push = 8;       // batched top-down levels with frontier arcs < all arcs / gamma push
                                 // path counts from the frontier (0: always pull from parents)
+  int pack_bu = 1;              // batched bottom-up packs vertices of degree <= 16 four to a wave
   int narrow_sigma = 1;         // exact batched sweeps first keep path counts in 32 bits (redone
                                 // with 64 bits when a count passes 2^32-1)
   int64_t gamma_back = 0;       // batched backward levels whose successor level holds fewer than

@@ -66,6 +66,11 @@ struct DevGraph {
   int64_t nall = 0;
   int32_t* split_v = nullptr;   // vertices held in more than one chunk record
   int64_t nsplit = 0;
+  int32_t* sm_v = nullptr;      // vertices of degree 1..16 (packed bottom-up)
+  int64_t nsm = 0;
+  int32_t* lg_v = nullptr;      // chunk records of the other vertices
+  int32_t* lg_j = nullptr;
+  int64_t nlg = 0;
   // peeling
   int32_t nrounds = 0;
   int32_t* round = nullptr;

@@ -195,6 +195,11 @@ struct GA {
   const int32_t* all_v;
   const int32_t* all_j;
   int64_t nall;
+  const int32_t* sm_v;  // vertices of degree 1..PACK_DEG (bottom-up packs PACK_W per wave)
+  int64_t nsm;
+  const int32_t* lg_v;  // chunk records of the other vertices
+  const int32_t* lg_j;
+  int64_t nlg;
   const int32_t* split_v;  // vertices held in more than one record
   int64_t nsplit;
   const uint64_t* w;
@@ -251,6 +256,7 @@ struct BA {
   int64_t alpha;
   int64_t beta;   // bottom-up only when the frontier holds >= 1/beta of all arcs
   int fuse;       // bottom-up levels also sum the path counts of unsplit vertices
+  int pack;       // bottom-up packs low-degree vertices PACK_W to a wave
   int64_t gamma;  // top-down levels whose frontier holds < 1/gamma of all arcs push path counts
   int64_t gamma_b;     // backward levels whose successor level holds < 1/gamma_b of their arcs are pushed
   int64_t* lvl_arcs;   // [level] degree sum of the level's vertices
@@ -942,7 +948,9 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 // the records of a split vertex (fuse_bu 2) add theirs atomically into its
 // row, whose columns are zero until the source reaches it.
 #define UNR_BU 8
-template <bool NAR>
+#define PACK_DEG 16  // packed bottom-up: vertices of degree <= PACK_DEG ...
+#define PACK_W 4     // ... PACK_W to a wave (PACK_DEG * PACK_W = 64)
+template <bool NAR, bool PACKED>
 __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone || c->mode != 1) return;
@@ -962,12 +970,17 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   int ovf = 0;
   double mx = 0.0;
   APPENDER(ap);
-  for (int64_t r = gw; r < g.nall; r += tw) {
-    int32_t v = g.all_v[r];
+  const bool pk = b.pack;
+  const int32_t* rv = pk ? g.lg_v : g.all_v;
+  const int32_t* rjv = pk ? g.lg_j : g.all_j;
+  const int64_t nr = pk ? g.nlg : g.nall;
+  if (!PACKED)
+  for (int64_t r = gw; r < nr; r += tw) {
+    int32_t v = rv[r];
     u64 sv = b.seen[v];
     u64 tgt = b.compmask[g.label[v]];
     if (sv == tgt) continue;
-    int j = g.all_j[r];
+    int j = rjv[r];
     const int64_t deg = g.off[v + 1] - g.off[v];
     int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
     int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
@@ -1051,6 +1064,106 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
     }
     app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
   }
+  if (PACKED) {
+    // Vertices of degree <= PACK_DEG, PACK_W per wave: sub-group q (lanes
+    // 16q..16q+15) probes the arcs of its vertex, one per lane; the path
+    // count sums then run with lanes = sources as above, one accumulator per
+    // sub-group, the parents of each vertex in adjacency order.
+    const int sub = lane / PACK_DEG, sl = lane % PACK_DEG;
+    const int64_t npk = (g.nsm + PACK_W - 1) / PACK_W;
+    for (int64_t p = gw; p < npk; p += tw) {
+      const int64_t idx = p * PACK_W + sub;
+      int32_t v = 0;
+      u64 sv = 0;
+      int64_t e0 = 0, dgv = 0;
+      if (idx < g.nsm) {
+        v = g.sm_v[idx];
+        sv = b.seen[v];
+        if (sv != b.compmask[g.label[v]]) {
+          e0 = g.off[v];
+          dgv = g.off[v + 1] - e0;
+        }
+      }
+      int32_t u = 0;
+      u64 need = 0;
+      if (sl < dgv) {
+        u = g.adj[e0 + sl];
+        need = cur[u] & ~sv;  // u is a parent of v for these sources
+      }
+      u64 acc = need;
+#pragma unroll
+      for (int o = PACK_DEG / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
+      u64 sq[PACK_W];  // per sub-group: exact sum, or FP64 bits when scaled
+#pragma unroll
+      for (int q = 0; q < PACK_W; q++) sq[q] = 0;
+      if (fuse) {
+        u64 P = __ballot(need != 0);
+        while (P) {
+          int qa[UNR_BU];
+          bool okq[UNR_BU];
+#pragma unroll
+          for (int t = 0; t < UNR_BU; t++) {
+            okq[t] = P != 0;
+            qa[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
+            if (okq[t]) P &= P - 1;
+          }
+          u64 x[UNR_BU];
+#pragma unroll
+          for (int t = 0; t < UNR_BU; t++) {
+            const int32_t uq = rl32(u, qa[t]);
+            const u64 nq = rl64(need, qa[t]);
+            const bool use = okq[t] && ((nq >> lane) & 1);
+            x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
+            ca += use;
+          }
+#pragma unroll
+          for (int t = 0; t < UNR_BU; t++) {
+            const int gq = qa[t] / PACK_DEG;  // wave-uniform
+#pragma unroll
+            for (int q = 0; q < PACK_W; q++) {
+              if (gq != q || !okq[t]) continue;
+              if (scaled) {
+                sq[q] = (u64)__double_as_longlong(__longlong_as_double((long long)sq[q]) +
+                                                  __longlong_as_double((long long)x[t]));
+              } else {
+                const u64 s2 = sq[q] + x[t];
+                if (s2 < sq[q]) ovf = 1;
+                sq[q] = s2;
+              }
+            }
+          }
+        }
+      }
+      bool won = false;
+      int64_t dg = 0;
+      if (sl == 0 && acc) {
+        if ((nxt[v] & acc) != acc) {
+          u64 old = atomicOr(&nxt[v], acc);
+          won = (old == 0);
+          if (won) {
+            dg = dgv;
+            if (g.flags & KF_GATE) lb_set(nlb, v);
+          }
+        }
+      }
+      if (fuse) {
+#pragma unroll
+        for (int q = 0; q < PACK_W; q++) {
+          const u64 aq = rl64(acc, q * PACK_DEG);
+          if (!((aq >> lane) & 1)) continue;
+          const int64_t row = (int64_t)rl32(v, q * PACK_DEG) * 64 + lane;
+          if (scaled) {
+            const double raw = __longlong_as_double((long long)sq[q]) * inv_m;
+            b.sigma[row] = (u64)__double_as_longlong(raw);
+            mx = fmax(mx, raw);
+          } else if (sg_put(b.sigma, nar, row, sq[q])) {
+            ovf = 1;
+          }
+        }
+      }
+      app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
+    }
+  }
   app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
   if (fuse) {
     block_add_lanes(ca, b.st_a);
@@ -1061,8 +1174,16 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
 
 __global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  if (b.ctl->narrow) k_ms_bu_body<true>(g, b);
-  else k_ms_bu_body<false>(g, b);
+  if (b.ctl->narrow) k_ms_bu_body<true, false>(g, b);
+  else k_ms_bu_body<false, false>(g, b);
+  kt_exit(kt);
+}
+
+// The packed low-degree vertices, in a kernel of their own (own register budget).
+__global__ void __launch_bounds__(256) k_ms_bu_pk(GA g, BA b, KT* kt) {
+  kt_enter(kt);
+  if (b.ctl->narrow) k_ms_bu_body<true, true>(g, b);
+  else k_ms_bu_body<false, true>(g, b);
   kt_exit(kt);
 }
 
@@ -1289,6 +1410,78 @@ __global__ void __launch_bounds__(256) k_ms_push(GA g, BA b) {
 // coef[v][i] = (delta+1)/sigma and bc[v] += sum_i mult_i * delta_i.  Also
 // publishes level d into lm[d%3] for the next (shallower) step and clears
 // level d+2 from lm[(d+2)%3].
+// Backward level d, one record (the pull): see k_ms_back_body.
+template <bool NAR>
+__device__ __forceinline__ void back_one(const GA& g, const BA& b, int64_t r, int d, const u64* lmn, u64* lmd,
+                                         const unsigned* nlb, unsigned* dlb, double mult, bool scaled,
+                                         double inv_md) {
+  const int lane = lane_id();
+  int32_t v = b.rec_v[r];
+  int j = b.rec_j[r];
+  u64 L = b.rec_mask[r - j];
+  int64_t deg = g.off[v + 1] - g.off[v];
+  int nch = nchunks(deg, g.chunk);
+  int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
+  int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+  const bool mine = (L >> lane) & 1;
+  const double wv = g.w ? (double)g.w[v] : 0.0;
+  const u64 sbits = mine ? sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane) : 1ull;
+  const double svr = !mine ? 1.0 : scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
+  const double sv = scaled ? svr * inv_md : svr;
+  double acc = (nch == 1) ? wv : 0.0;
+  for (int64_t eb = e0; eb < e1; eb += 64) {
+    int64_t e = eb + lane;
+    int32_t z = 0;
+    u64 M = 0;
+    if (e < e1) {
+      z = adj_at(g, g.adj, e);
+      if (!(g.flags & KF_GATE) || lb_test(nlb, z)) M = lmn[z] & L;
+    }
+    u64 P = __ballot(M != 0);
+    // Successors UNR at a time (loads first, then the sums in adjacency
+    // order; FP contraction is off, so each step rounds like the oracle).
+    while (P) {
+      int q[UNR];
+      bool ok[UNR];
+#pragma unroll
+      for (int t = 0; t < UNR; t++) {
+        ok[t] = P != 0;
+        q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
+        if (ok[t]) P &= P - 1;
+      }
+      double x[UNR];
+      bool use[UNR];
+#pragma unroll
+      for (int t = 0; t < UNR; t++) {
+        int32_t zq = rl32(z, q[t]);
+        u64 mq = rl64(M, q[t]);
+        use[t] = ok[t] && ((mq >> lane) & 1);
+        x[t] = use[t] ? b.coef[(int64_t)zq * 64 + lane] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < UNR; t++)
+        if (use[t]) acc = acc + x[t] * sv;
+    }
+  }
+  double* crow = b.coef + (int64_t)v * 64 + lane;
+  if (nch > 1) {
+    if (mine) atomicAdd(crow, acc);  // finished by k_ms_bfin
+  } else {
+    double contrib = 0.0;
+    if (mine) {
+      *crow = (acc + 1.0) / svr;
+      contrib = mult * acc;
+      if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = acc;
+    }
+    contrib = wave_sum_d(contrib);
+    if (lane == 0) b.bc[v] += contrib;
+  }
+  if (j == 0 && lane == 0) {
+    lmd[v] = L;
+    if (g.flags & KF_GATE) lb_set(dlb, v);
+  }
+}
+
 template <bool NAR>
 __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   Ctl* c = b.ctl;
@@ -1307,84 +1500,20 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   // level d and w on level d+1, so the terms use raw(v)/m(d) and coef keeps raw.
   const bool scaled = c->scaled;
   const double inv_md = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
-  const bool bpush = c->bpush;
-  for (int64_t r = lo + gw; r < hi; r += tw) {
-    int32_t v = b.rec_v[r];
-    int j = b.rec_j[r];
-    if (bpush) {
-      // pushed level: only zero the accumulators and publish the level
-      if (j == 0) {
-        u64 L = b.rec_mask[r];
-        if ((L >> lane) & 1) b.coef[(int64_t)v * 64 + lane] = 0.0;
-        if (lane == 0) {
-          lmd[v] = L;
-          if (g.flags & KF_GATE) lb_set(dlb, v);
-        }
-      }
-      continue;
-    }
-    u64 L = b.rec_mask[r - j];
-    int64_t deg = g.off[v + 1] - g.off[v];
-    int nch = nchunks(deg, g.chunk);
-    int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
-    int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
-    const bool mine = (L >> lane) & 1;
-    const double wv = g.w ? (double)g.w[v] : 0.0;
-    const u64 sbits = mine ? sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane) : 1ull;
-    const double svr = !mine ? 1.0 : scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
-    const double sv = scaled ? svr * inv_md : svr;
-    double acc = (nch == 1) ? wv : 0.0;
-    for (int64_t eb = e0; eb < e1; eb += 64) {
-      int64_t e = eb + lane;
-      int32_t z = 0;
-      u64 M = 0;
-      if (e < e1) {
-        z = adj_at(g, g.adj, e);
-        if (!(g.flags & KF_GATE) || lb_test(nlb, z)) M = lmn[z] & L;
-      }
-      u64 P = __ballot(M != 0);
-      // Successors UNR at a time (loads first, then the sums in adjacency
-      // order; FP contraction is off, so each step rounds like the oracle).
-      while (P) {
-        int q[UNR];
-        bool ok[UNR];
-#pragma unroll
-        for (int t = 0; t < UNR; t++) {
-          ok[t] = P != 0;
-          q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
-          if (ok[t]) P &= P - 1;
-        }
-        double x[UNR];
-        bool use[UNR];
-#pragma unroll
-        for (int t = 0; t < UNR; t++) {
-          int32_t zq = rl32(z, q[t]);
-          u64 mq = rl64(M, q[t]);
-          use[t] = ok[t] && ((mq >> lane) & 1);
-          x[t] = use[t] ? b.coef[(int64_t)zq * 64 + lane] : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < UNR; t++)
-          if (use[t]) acc = acc + x[t] * sv;
+  if (c->bpush) {
+    // pushed level: only zero the accumulators and publish the level
+    for (int64_t r = lo + gw; r < hi; r += tw) {
+      if (b.rec_j[r] != 0) continue;
+      const int32_t v = b.rec_v[r];
+      const u64 L = b.rec_mask[r];
+      if ((L >> lane) & 1) b.coef[(int64_t)v * 64 + lane] = 0.0;
+      if (lane == 0) {
+        lmd[v] = L;
+        if (g.flags & KF_GATE) lb_set(dlb, v);
       }
     }
-    double* crow = b.coef + (int64_t)v * 64 + lane;
-    if (nch > 1) {
-      if (mine) atomicAdd(crow, acc);  // finished by k_ms_bfin
-    } else {
-      double contrib = 0.0;
-      if (mine) {
-        *crow = (acc + 1.0) / svr;
-        contrib = mult * acc;
-        if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = acc;
-      }
-      contrib = wave_sum_d(contrib);
-      if (lane == 0) b.bc[v] += contrib;
-    }
-    if (j == 0 && lane == 0) {
-      lmd[v] = L;
-      if (g.flags & KF_GATE) lb_set(dlb, v);
-    }
+  } else {
+    for (int64_t r = lo + gw; r < hi; r += tw) back_one<NAR>(g, b, r, d, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
   }
   {
     u64* old = b.lm + (int64_t)((d + 2) % 3) * b.n;
@@ -2007,13 +2136,13 @@ void DevGraph::release() {
   auto fr = [](void* p) { if (p) (void)hipFree(p); };
   if (toff != off) { fr(toff); fr(tadj); }
   if (off != foff) { fr(off); fr(adj); }
-  fr(foff); fr(fadj); fr(label); fr(comp_size); fr(comp_arcs); fr(all_v); fr(all_j); fr(split_v);
+  fr(foff); fr(fadj); fr(label); fr(comp_size); fr(comp_arcs); fr(all_v); fr(all_j); fr(split_v); fr(sm_v); fr(lg_v); fr(lg_j);
   fr(round); fr(parent); fr(w); fr(root); fr(order); fr(d_n2o);
   d_n2o = nullptr; relabeled = false; h_n2o.clear(); h_o2n.clear();
   sigma_overflowed.store(false);
   narrow_overflowed.store(false);
   off = foff = toff = nullptr; adj = fadj = tadj = nullptr; label = nullptr; comp_size = comp_arcs = nullptr;
-  all_v = all_j = split_v = nullptr; nsplit = 0; round = parent = root = order = nullptr; w = nullptr;
+  all_v = all_j = split_v = sm_v = lg_v = lg_j = nullptr; nsplit = nsm = nlg = 0; round = parent = root = order = nullptr; w = nullptr;
   n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); h_sig.clear(); bytes = 0;
 }
 
@@ -2238,6 +2367,26 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
     HIP_TRY(hipStreamSynchronize(st));
   }
   {
+    // packed bottom-up lists: low-degree vertices, and the records of the rest
+    std::vector<int32_t> smv, lgv, lgj;
+    for (size_t r = 0; r < rv.size(); r++) {
+      const int64_t dg = hoff[rv[r] + 1] - hoff[rv[r]];
+      if (dg <= PACK_DEG) smv.push_back(rv[r]);
+      else { lgv.push_back(rv[r]); lgj.push_back(rj[r]); }
+    }
+    g.nsm = (int64_t)smv.size();
+    g.nlg = (int64_t)lgv.size();
+    TRY(dalloc(&g.sm_v, std::max<int64_t>(g.nsm, 1), &acc));
+    TRY(dalloc(&g.lg_v, std::max<int64_t>(g.nlg, 1), &acc));
+    TRY(dalloc(&g.lg_j, std::max<int64_t>(g.nlg, 1), &acc));
+    if (g.nsm) HIP_TRY(hipMemcpyAsync(g.sm_v, smv.data(), sizeof(int32_t) * g.nsm, hipMemcpyHostToDevice, st));
+    if (g.nlg) {
+      HIP_TRY(hipMemcpyAsync(g.lg_v, lgv.data(), sizeof(int32_t) * g.nlg, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(g.lg_j, lgj.data(), sizeof(int32_t) * g.nlg, hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  {
     std::vector<int32_t> sv;
     for (size_t r = 0; r < rv.size(); r++)
       if (rj[r] == 1) sv.push_back(rv[r]);
@@ -2382,6 +2531,7 @@ static GA make_ga(const DevGraph& g, const Params& prm) {
   a.off = g.off; a.adj = g.adj; a.toff = g.toff; a.tadj = g.tadj; a.directed = g.directed;
   a.label = g.label; a.comp_arcs = g.comp_arcs;
   a.all_v = g.all_v; a.all_j = g.all_j; a.nall = g.nall; a.split_v = g.split_v; a.nsplit = g.nsplit;
+  a.sm_v = g.sm_v; a.nsm = g.nsm; a.lg_v = g.lg_v; a.lg_j = g.lg_j; a.nlg = g.nlg;
   a.w = g.heuristic ? (const uint64_t*)g.w : nullptr;
   a.chunk = g.chunk; a.n = g.n; a.m = g.m;
   a.flags = prm.kflags;
@@ -2401,7 +2551,7 @@ static BA make_ba(Workspace* ws, const Params& prm) {
   b.compmask = ws->compmask; b.srcbuf = ws->srcbuf; b.multbuf = ws->multbuf; b.bc = ws->bc;
   b.st_n = ws->stats; b.st_m = ws->stats + 64; b.st_a = ws->stats + 128; b.dbg_delta = ws->dbg_delta;
   b.alpha = prm.alpha_batched; b.beta = prm.beta_batched; b.gamma = prm.gamma_push; b.fuse = prm.fuse_bu;
-  b.gamma_b = prm.gamma_back; b.lvl_arcs = ws->lvl_arcs; b.n = ws->n;
+  b.gamma_b = prm.gamma_back; b.lvl_arcs = ws->lvl_arcs; b.pack = prm.pack_bu; b.n = ws->n;
   b.lscale = ws->lscale; b.lscale_cap = ws->lscale_cap; b.lvmax = ws->lvmax;
   return b;
 }
@@ -2623,7 +2773,7 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
   for (int which = 0; which < 4; which++) {
     const bool fwd = (which & 1) == 0;
     const bool sparse = which >= 2;
-    const int Gk = sparse ? prm.sgrid : plain ? prm.grid_plain : G;
+    const int Gk = sparse ? prm.sgrid : plain ? prm.grid_plain : fwd || prm.grid_back <= 0 ? G : prm.grid_back;
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < prm.steps; s++) {
@@ -2649,6 +2799,10 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
           ktacc(3, Gk);
           if (!sparse) hipLaunchKernelGGL(k_ms_bu, Gk, B, 0, st, ga, ba, tk(4));
           if (!sparse) ktacc(4, Gk);
+          if (!sparse && prm.pack_bu) {
+            hipLaunchKernelGGL(k_ms_bu_pk, Gk, B, 0, st, ga, ba, tk(4));
+            ktacc(4, Gk);
+          }
           hipLaunchKernelGGL(k_ms_sigma, Gk, B, 0, st, ga, ba, tk(1));
           ktacc(1, Gk);
           hipLaunchKernelGGL(k_ms_push, Gk, B, 0, st, ga, ba);
@@ -2684,10 +2838,11 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
 }
 
 static bool same_launch(const Params& a, const Params& b) {
-  return a.block == b.block && a.grid == b.grid && a.grid_plain == b.grid_plain && a.chunk == b.chunk &&
+  return a.block == b.block && a.grid == b.grid && a.grid_plain == b.grid_plain && a.grid_back == b.grid_back &&
+         a.chunk == b.chunk &&
          a.alpha_plain == b.alpha_plain &&
          a.alpha_batched == b.alpha_batched && a.beta_plain == b.beta_plain && a.beta_batched == b.beta_batched &&
-         a.gamma_push == b.gamma_push && a.gamma_back == b.gamma_back && a.fuse_bu == b.fuse_bu &&
+         a.gamma_push == b.gamma_push && a.gamma_back == b.gamma_back && a.fuse_bu == b.fuse_bu && a.pack_bu == b.pack_bu &&
          a.steps == b.steps && a.ktime == b.ktime &&
          a.kflags == b.kflags && a.sgrid == b.sgrid;
 }

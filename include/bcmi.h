@@ -125,6 +125,9 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * "beta_batched" (and only when frontier arcs >= all arcs / beta),
  * "fuse_bu" (bottom-up levels also sum path counts in their scan: 0 never,
  * 1 for vertices held in one work record, 2 for all - the default),
+ * "grid_back" (blocks per batched backward kernel; 0, the default, uses
+ * "grid"), "pack_bu" (1, the default: batched bottom-up levels scan vertices of
+ * degree <= 16 four to a wave in a kernel of their own; 0: one per wave),
  * "narrow_sigma" (1, the default: exact batched sweeps first keep path
  * counts in 32-bit rows and redo a batch with 64-bit rows when a count
  * passes 2^32-1; 0: 64-bit rows only),

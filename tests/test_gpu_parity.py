@@ -300,20 +300,21 @@ def test_degenerate_inputs(mode, strategy):
 
 
 @pytest.mark.parametrize("sigma", [2, 1])
-@pytest.mark.parametrize("gamma_back,fuse_bu,chunk", [(0, 1, 256), (0, 0, 64), (2, 2, 256), (8, 2, 64),
-                                                      (1 << 40, 1, 64)])
-def test_batched_level_variants(gamma_back, fuse_bu, chunk, sigma):
+@pytest.mark.parametrize("gamma_back,fuse_bu,chunk,pack_bu", [(0, 1, 256, 1), (0, 0, 64, 1), (2, 2, 256, 0),
+                                                              (8, 2, 64, 1), (1 << 40, 1, 64, 0), (0, 2, 64, 0)])
+def test_batched_level_variants(gamma_back, fuse_bu, chunk, pack_bu, sigma):
     """Bottom-up fusion (none, unsplit vertices, all: split vertices add
     partial sums into rows kept zero between sweeps) and pushed backward
-    levels (gamma_back; 2^40 pushes every level with a successor level), with
-    records of 64 and 256 arcs, exact and scaled path counts: BC within 1e-9
-    of the oracle, twice in a row on the same engine."""
+    levels (gamma_back; 2^40 pushes every level with a successor level),
+    bottom-up with and without the packed low-degree kernel, with records of
+    64 and 256 arcs, exact and scaled path counts: BC within 1e-9 of the
+    oracle, twice in a row on the same engine."""
     graphs = [bcmi.Graph.rmat(12, 16, seed=3), bcmi.Graph.block_chain(40, 128, 6.0, seed=5)]
     for g in graphs:
         src = g.sample_sources(200, seed=11)
         ref = O.brandes(g.off, g.adj, src, nthreads=8)
         for h in (bcmi.H_NONE, bcmi.H_PEEL):
-            e = bcmi.Engine(gamma_back=gamma_back, fuse_bu=fuse_bu, chunk=chunk, sigma=sigma)
+            e = bcmi.Engine(gamma_back=gamma_back, fuse_bu=fuse_bu, chunk=chunk, sigma=sigma, pack_bu=pack_bu)
             e.upload(g, heuristic=h)
             for _ in range(2):
                 e.reset()
