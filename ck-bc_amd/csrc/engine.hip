@@ -1213,6 +1213,52 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   constexpr bool nar = NAR;
   const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   double mx = 0.0;
+  // After a bottom-up level that summed every vertex's parents, or before a
+  // push (which only needs zeroed rows), no parents are scanned: the
+  // bookkeeping runs one lane per record, 64 records per wave.
+  const bool book = (push || (fused && b.fuse > 1)) && b.pack;
+  if (book) {
+    for (int64_t base = lo + gw * 64; base < hi; base += tw * 64) {
+      const int64_t r = base + lane;
+      int32_t v = 0;
+      bool first = false;
+      if (r < hi) {
+        v = b.rec_v[r];
+        first = b.rec_j[r] == 0;
+      }
+      u64 N = 0;
+      int64_t deg = 0;
+      if (first) {
+        N = nxt[v];
+        deg = g.off[v + 1] - g.off[v];
+        const u64 so = b.seen[v];
+        const u64 sn = so | N;
+        b.seen[v] = sn;
+        b.rec_mask[r] = N;
+        if (so == 0) { uv++; ua += deg; }
+        if (sn == b.compmask[g.label[v]]) done_arcs += deg;
+      }
+      // per-source counts (lanes = sources again) and the backward
+      // accumulators of split vertices
+      u64 F = __ballot(first);
+      while (F) {
+        const int q = __ffsll((long long)F) - 1;
+        F &= F - 1;
+        const u64 Nq = rl64(N, q);
+        const int64_t dq = (int64_t)rl64((u64)deg, q);
+        if ((Nq >> lane) & 1) {
+          cn++;
+          cm += dq;
+          const int64_t row = (int64_t)rl32(v, q) * 64 + lane;
+          if (dq > g.chunk) b.coef[row] = 0.0;
+          if (push) sg_st(b.sigma, nar, row, 0);  // k_ms_push adds the parents' counts next
+        }
+      }
+    }
+    done_arcs = wave_sum_ll(done_arcs);
+    uv = wave_sum_ll(uv);
+    ua = wave_sum_ll(ua);
+  } else
   for (int64_t r = lo + gw; r < hi; r += tw) {
     int32_t v = b.rec_v[r];
     int j = b.rec_j[r];
@@ -1588,6 +1634,29 @@ __global__ void __launch_bounds__(256) k_ms_bpush(GA g, BA b, KT* kt) {
 // (delta = w(v) + the atomically added partial sums held in coef) or, on a
 // pushed level, every level-d vertex (delta = w(v) + sigma(v) x the pushed
 // sum of its successors' coef).
+// One such vertex (wave-wide; lanes = sources).
+template <bool NAR>
+__device__ __forceinline__ void bfin_one(const GA& g, const BA& b, int64_t r, int32_t v, bool bpush, bool scaled,
+                                         double inv_md, double mult) {
+  const int lane = lane_id();
+  const u64 L = b.rec_mask[r];
+  const bool mine = (L >> lane) & 1;
+  double contrib = 0.0;
+  if (mine) {
+    double* crow = b.coef + (int64_t)v * 64 + lane;
+    u64 sbits = sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane);
+    double svr = scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
+    double part = *crow;
+    if (bpush) part = part * (scaled ? svr * inv_md : svr);
+    double dl = (g.w ? (double)g.w[v] : 0.0) + part;
+    *crow = (dl + 1.0) / svr;
+    contrib = mult * dl;
+    if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = dl;
+  }
+  contrib = wave_sum_d(contrib);
+  if (lane == 0) b.bc[v] += contrib;
+}
+
 template <bool NAR>
 __device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
   Ctl* c = b.ctl;
@@ -1601,27 +1670,26 @@ __device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
   const bool bpush = c->bpush;
   const bool scaled = c->scaled;
   const double inv_md = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
-  for (int64_t r = lo + gw; r < hi; r += tw) {
-    if (b.rec_j[r] != 0) continue;
-    int32_t v = b.rec_v[r];
-    int64_t deg = g.off[v + 1] - g.off[v];
-    if (deg <= g.chunk && !bpush) continue;
-    u64 L = b.rec_mask[r];
-    const bool mine = (L >> lane) & 1;
-    double contrib = 0.0;
-    if (mine) {
-      double* crow = b.coef + (int64_t)v * 64 + lane;
-      u64 sbits = sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane);
-      double svr = scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
-      double part = *crow;
-      if (bpush) part = part * (scaled ? svr * inv_md : svr);
-      double dl = (g.w ? (double)g.w[v] : 0.0) + part;
-      *crow = (dl + 1.0) / svr;
-      contrib = mult * dl;
-      if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = dl;
+  if (bpush) {
+    for (int64_t r = lo + gw; r < hi; r += tw)
+      if (b.rec_j[r] == 0) bfin_one<NAR>(g, b, r, b.rec_v[r], true, scaled, inv_md, mult);
+    return;
+  }
+  // only split vertices are left: find their first records 64 at a time
+  for (int64_t base = lo + gw * 64; base < hi; base += tw * 64) {
+    const int64_t r = base + lane;
+    int32_t v = 0;
+    bool split = false;
+    if (r < hi && b.rec_j[r] == 0) {
+      v = b.rec_v[r];
+      split = g.off[v + 1] - g.off[v] > g.chunk;
     }
-    contrib = wave_sum_d(contrib);
-    if (lane == 0) b.bc[v] += contrib;
+    u64 S = __ballot(split);
+    while (S) {
+      const int q = __ffsll((long long)S) - 1;
+      S &= S - 1;
+      bfin_one<NAR>(g, b, base + q, rl32(v, q), false, scaled, inv_md, mult);
+    }
   }
 }
 
