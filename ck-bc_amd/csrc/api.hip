@@ -168,7 +168,7 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
     if (v < 0 || v > 3) return set_error(BCMI_ERR_ARG, "kflags must be 0..3");
     p.kflags = (int)v;
   } else if (k == "batching") {
-    if (v < 0 || v > 3) return set_error(BCMI_ERR_ARG, "batching must be 0..3");
+    if (v < 0 || v > 5) return set_error(BCMI_ERR_ARG, "batching must be 0..5");
     p.batching = (int)v;
   } else if (k == "reorder") {
     if (c->have_graph) return set_error(BCMI_ERR_STATE, "reorder must be set before uploading a graph");

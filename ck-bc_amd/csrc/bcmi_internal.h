@@ -53,8 +53,10 @@ struct Params {
                             // 2 exact, batches that overflow are recomputed scaled
   int kflags = 0;           // kernel variants: 1 gate mask probes with level bitmaps,
                             // 2 non-temporal adjacency loads
-  int batching = 2;         // source order for batches: 0 as given, 1 hub-depth signature,
-                            // 2 highest-degree neighbour then signature
+  int batching = 4;         // source order for batches: 0 as given, 1 hub-depth signature,
+                            // 2 highest-degree neighbour then signature, 3 that neighbour's
+                            // signature first, 4 highest then second highest-degree neighbour
+                            // then signature, 5 highest-degree neighbour, signature, second
   int reorder = 1;          // relabel vertices by descending degree at upload (locality)
   int trace = 0;            // diagnostic: per-level statistics on stderr (use with steps = 1)
   int ktime = 0;            // kernel timed on the device (wave entry/exit stamps) inside the level loop:

@@ -83,6 +83,7 @@ struct DevGraph {
   std::vector<int32_t> h_round;  // removal round per vertex on the host
   std::vector<uint32_t> h_sig;   // batching key: BFS depths from the four largest hubs
   std::vector<int32_t> h_topnb;  // batching key: highest-degree neighbour
+  std::vector<int32_t> h_topnb2; // batching key: second highest-degree neighbour
   int depth_hint = 0;            // deepest level of the hubs' BFS
   int cc_passes = 0;             // union-find passes the component labels took
   // Vertices are relabelled by descending degree on upload (hubs' rows and

@@ -141,14 +141,18 @@ __device__ __forceinline__ int nchunks(int64_t deg, int chunk) {
 
 // Device-side kernel timer on the constant-rate wall clock (s_memrealtime):
 // every wave of the timed kernel stamps its entry and exit into its own slot
-// (plain stores, no shared counters), and a one-block kernel launched right
-// after it adds (last exit - first entry) to the total.
+// (plain stores, no shared counters).  Right after it, k_kt_accum (KT_PARTS
+// blocks) reduces slices of the stamps to partial (first entry, last exit)
+// pairs and the one-wave k_kt_fold adds last exit - first entry to the total.
+#define KT_PARTS 64
 struct KT {
   u64 total, launches;
   u64* wstart;  // per wave of the launch
   u64* wend;
+  u64 plo[KT_PARTS], phi[KT_PARTS];  // partial first entry / last exit per accumulating block
 };
 #define KT_MAX_WAVES (65536 * 4)
+static_assert(KT_PARTS == 64, "k_kt_fold folds the partials in one wave");
 __device__ __forceinline__ int64_t kt_wave() {
   return ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
 }
@@ -158,10 +162,11 @@ __device__ __forceinline__ void kt_enter(KT* kt) {
 __device__ __forceinline__ void kt_exit(KT* kt) {
   if (kt && __lane_id() == 0) kt->wend[kt_wave()] = (u64)wall_clock64();
 }
+// grid: KT_PARTS blocks of 1024 threads
 __global__ void __launch_bounds__(1024) k_kt_accum(KT* kt, int64_t nwaves) {
   __shared__ u64 s_lo[16], s_hi[16];
   u64 lo = ~0ull, hi = 0;
-  for (int64_t i = threadIdx.x; i < nwaves; i += blockDim.x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwaves; i += (int64_t)gridDim.x * blockDim.x) {
     u64 a = kt->wstart[i], b = kt->wend[i];
     lo = a < lo ? a : lo;
     hi = b > hi ? b : hi;
@@ -178,6 +183,19 @@ __global__ void __launch_bounds__(1024) k_kt_accum(KT* kt, int64_t nwaves) {
       lo = s_lo[w] < lo ? s_lo[w] : lo;
       hi = s_hi[w] > hi ? s_hi[w] : hi;
     }
+    kt->plo[blockIdx.x] = lo;
+    kt->phi[blockIdx.x] = hi;
+  }
+}
+// grid: one block of KT_PARTS (= 64) threads, after k_kt_accum
+__global__ void __launch_bounds__(64) k_kt_fold(KT* kt) {
+  u64 lo = kt->plo[threadIdx.x], hi = kt->phi[threadIdx.x];
+  for (int o = 32; o > 0; o >>= 1) {
+    u64 a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if (threadIdx.x == 0) {
     if (hi > lo) kt->total += hi - lo;
     kt->launches += 1;
   }
@@ -2415,9 +2433,19 @@ int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* ad
   g.h_off = hoff;
   hub_signature(n, off, adj, g.h_sig, &g.depth_hint);
   // first neighbour in descending-degree order (the smallest internal id once relabelled)
+  // and the second one (INT32_MAX for vertices of degree < 2)
   g.h_topnb.assign(n, INT32_MAX);
+  g.h_topnb2.assign(n, INT32_MAX);
   for (int64_t v = 0; v < n; v++)
-    for (int64_t e = off[v]; e < off[v + 1]; e++) g.h_topnb[v] = std::min(g.h_topnb[v], adj[e]);
+    for (int64_t e = off[v]; e < off[v + 1]; e++) {
+      const int32_t u = adj[e];
+      if (u < g.h_topnb[v]) {
+        g.h_topnb2[v] = g.h_topnb[v];
+        g.h_topnb[v] = u;
+      } else if (u < g.h_topnb2[v]) {
+        g.h_topnb2[v] = u;
+      }
+    }
   g.h_round.assign(n, 0);
   if (heuristic) {
     HIP_TRY(hipMemcpyAsync(g.h_round.data(), g.round, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
@@ -2833,7 +2861,9 @@ static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
   auto tk = [&](int id) -> KT* { return kt == id ? T : nullptr; };
   // after a timed kernel of Gk blocks: fold its wave stamps into the total
   auto ktacc = [&](int id, int Gk) {
-    if (kt == id) hipLaunchKernelGGL(k_kt_accum, 1, 1024, 0, st, T, (int64_t)Gk * B / 64);
+    if (kt != id) return;
+    hipLaunchKernelGGL(k_kt_accum, KT_PARTS, 1024, 0, st, T, (int64_t)Gk * B / 64);
+    hipLaunchKernelGGL(k_kt_fold, 1, KT_PARTS, 0, st, T);
   };
   // Four graphs: {large grid, small grid} x {forward, backward}.  The small
   // one serves levels with few vertices (long chains): its launches are cheap
@@ -3391,6 +3421,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
     for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
     const std::vector<uint32_t>& sg = g.h_sig;
     const std::vector<int32_t>& tn = g.h_topnb;
+    const std::vector<int32_t>& tn2 = g.h_topnb2;
     if (prm.batching == 3)
       std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
         int32_t ta = tn[cs[a]], tb = tn[cs[b]];
@@ -3398,6 +3429,21 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
         if (ka != kb) return ka < kb;
         if (ta != tb) return ta < tb;
         return sg[cs[a]] < sg[cs[b]];
+      });
+    else if (prm.batching == 4)
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+        if (ta != tb) return ta < tb;
+        int32_t ua = tn2[cs[a]], ub = tn2[cs[b]];
+        if (ua != ub) return ua < ub;
+        return sg[cs[a]] < sg[cs[b]];
+      });
+    else if (prm.batching == 5)
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+        if (ta != tb) return ta < tb;
+        if (sg[cs[a]] != sg[cs[b]]) return sg[cs[a]] < sg[cs[b]];
+        return tn2[cs[a]] < tn2[cs[b]];
       });
     else if (prm.batching == 2)
       std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {

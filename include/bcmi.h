@@ -142,7 +142,10 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * uint64, BCMI_ERR_OVERFLOW past 2^64-1; 1 FP64 scaled per level; 2 exact,
  * with batches that overflow recomputed scaled - the default; the plain
  * sweep is exact only), "batching" (source order: 0 as given, 1 hub-depth
- * signature, 2 highest-degree neighbour then signature), "reorder"
+ * signature, 2 highest-degree neighbour then signature, 3 that neighbour's
+ * signature, the neighbour, the signature, 4 - the default - highest then
+ * second highest-degree neighbour then signature, 5 highest-degree
+ * neighbour, signature, second neighbour), "reorder"
  * (relabel by degree at upload), "kflags" (kernel variants), "sgrid"
  * (blocks per kernel on small levels), "sparse_verts" (levels with fewer
  * vertices use the small grid), "streams" (sweeps of one call run at once,
