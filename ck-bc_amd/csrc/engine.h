@@ -81,7 +81,7 @@ struct DevGraph {
   std::vector<int64_t> round_lo; // offsets of each round in order (size nrounds+1)
   std::vector<int64_t> h_off;    // traversed offsets on the host (degrees)
   std::vector<int32_t> h_round;  // removal round per vertex on the host
-  std::vector<uint32_t> h_sig;   // batching key: BFS depths from the four largest hubs
+  std::vector<uint64_t> h_sig;   // batching key: BFS depths from the eight largest hubs
   std::vector<int32_t> h_topnb;  // batching key: highest-degree neighbour
   std::vector<int32_t> h_topnb2; // batching key: second highest-degree neighbour
   int depth_hint = 0;            // deepest level of the hubs' BFS

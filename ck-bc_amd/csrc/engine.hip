@@ -2233,12 +2233,14 @@ void DevGraph::release() {
 }
 
 // Batching key per vertex: BFS depths (8 bits each, 255 = unreached) from the
-// four highest-degree vertices.  Sources with equal keys have similar depth
-// profiles, so a batch of them shares levels: its per-vertex masks are dense
-// and each vertex appears on few levels.
-static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std::vector<uint32_t>& sig,
+// H highest-degree vertices, the largest hub in the most significant byte.
+// Sources with equal keys have similar depth profiles, so a batch of them
+// shares levels: its per-vertex masks are dense and each vertex appears on
+// few levels.  max_depth: deepest level seen from the first HD hubs (sizing
+// hint for the work list and the engine choice).
+static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std::vector<uint64_t>& sig,
                           int* max_depth) {
-  const int H = 4;
+  const int H = 8, HD = 4;
   std::vector<int32_t> hubs;
   {
     std::vector<std::pair<int64_t, int32_t>> best;
@@ -2281,10 +2283,10 @@ static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std
     });
   for (auto& t : th) t.join();
   *max_depth = 0;
-  for (int h = 0; h < (int)hubs.size(); h++) *max_depth = std::max(*max_depth, hmax[h]);
+  for (int h = 0; h < std::min(HD, (int)hubs.size()); h++) *max_depth = std::max(*max_depth, hmax[h]);
   sig.assign(n, 0);
   for (int64_t v = 0; v < n; v++) {
-    uint32_t k = 0;
+    uint64_t k = 0;
     for (int h = 0; h < H; h++) k = (k << 8) | (h < (int)hubs.size() ? dep[h][v] : 0);
     sig[v] = k;
   }
@@ -3419,7 +3421,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
     // sources, so the order only changes the rounding of the final sums).
     std::vector<size_t> idx(cs.size());
     for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
-    const std::vector<uint32_t>& sg = g.h_sig;
+    const std::vector<uint64_t>& sg = g.h_sig;
     const std::vector<int32_t>& tn = g.h_topnb;
     const std::vector<int32_t>& tn2 = g.h_topnb2;
     if (prm.batching == 3)
