@@ -6,6 +6,8 @@ source equals the sum over its reached targets of d(s,t) - 1), which holds
 whatever the size of the path counts.
 
   * R-MAT scale 18 + 30% pendant trees (n = 340,787), 8,192 sources;
+  * R-MAT scale 22 (4.2M vertices, ~128M arcs), all 16,384 sources with the
+    launch shape tuned for it;
   * R-MAT scale 25 (33.5M vertices, ~1.07G arcs);
   * a bridged-block chain of 32,768 blocks of 256 vertices (8.4M vertices,
     path counts far beyond 2^64, tens of thousands of levels).
@@ -68,6 +70,53 @@ def test_pendant_trees_scale18():
     e.run(sub, mode=bcmi.MODE_PLAIN)
     ref = O.bc_pipeline(g.off, g.adj, sub, heuristic=1, nthreads=NTH)
     assert bcmi.rel_err(e.bc(), ref) <= 1e-9
+    e.close()
+
+
+def test_rmat_scale22_all_sources_tuned():
+    """The batched workload as measured: all 16,384 sources in batches of 64
+    with the tuned launch shape.  BC is linear in the sources, so the whole
+    run equals the sum of two runs over its halves (batched differently);
+    a 512-source run matches the oracle's distance checksum; two sources
+    match the CPU oracle; plain and batched sweeps agree on 64 sources."""
+    import autotune
+    t = time.time()
+    g = bcmi.Graph.rmat(22, 16, seed=SEED)
+    src = g.sample_sources(16384, seed=SEED + 1)
+    say(f"rmat22: n={g.n} arcs={g.m} built in {time.time() - t:.1f}s")
+    e = bcmi.Engine()
+    assert autotune.apply(e, autotune.load(), autotune.graph_key("rmat", scale=22, edgefactor=16))
+    e.upload(g)
+    e.reset()
+    st = e.run(src, mode=bcmi.MODE_BATCHED)
+    assert st["batches"] == 256
+    full = e.bc()
+    half = np.zeros_like(full)
+    for part in (src[:8192], src[8192:]):
+        e.reset()
+        e.run(part, mode=bcmi.MODE_BATCHED)
+        half += e.bc()
+    assert bcmi.rel_err(full, half) <= 1e-9
+    say("rmat22: all sources = sum of the halves")
+    sub = src[:512]
+    e.reset()
+    e.run(sub, mode=bcmi.MODE_BATCHED)
+    ds = O.depth_sums(g.off, g.adj, sub, nthreads=NTH)
+    assert rel_sum(e.bc().sum(), ds.sum()) <= 1e-9
+    say("rmat22: 512-source checksum done")
+    two = src[:2]
+    e.reset()
+    e.run(two, mode=bcmi.MODE_BATCHED)
+    ref = O.brandes(g.off, g.adj, two, nthreads=2)
+    assert bcmi.rel_err(e.bc(), ref) <= 1e-9
+    sixty4 = src[:64]
+    e.reset()
+    e.run(sixty4, mode=bcmi.MODE_BATCHED)
+    bb = e.bc()
+    e.reset()
+    e.run(sixty4, mode=bcmi.MODE_PLAIN)
+    assert bcmi.rel_err(bb, e.bc()) <= 1e-9
+    say("rmat22: oracle and plain-vs-batched done")
     e.close()
 
 
