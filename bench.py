@@ -18,7 +18,8 @@ committed rocprofv3 counter runs, the CPU oracle timed on a bounded sample
 and a served-query run (p50/p99 latency).
 
 Other workloads (not the default line):
-  --scale 18 --pendant 0.3 --sources 8192     R-MAT 18 + 30% pendant trees, peeled
+  --scale 18 --pendant 0.3 --sources 8192 --steps 126
+                                              R-MAT 18 + 30% pendant trees, peeled
   --scale 25 --sources 4096 --steps 62        R-MAT scale 25
   --graph chain --chain-blocks 32768 --sources 1024 --steps 12 --warmup 4 --kernel none --wg-mem 192
                                               bridged-block chain, 8.4M vertices

@@ -968,6 +968,7 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 #define UNR_BU 8
 #define PACK_DEG 16  // packed bottom-up: vertices of degree <= PACK_DEG ...
 #define PACK_W 4     // ... PACK_W to a wave (PACK_DEG * PACK_W = 64)
+static_assert(PACK_DEG * PACK_W == 64 && PACK_W == 4, "the packed bottom-up scan picks one vertex per sub-group of 16 lanes");
 template <bool NAR, bool PACKED>
 __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   Ctl* c = b.ctl;
@@ -992,95 +993,112 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   const int32_t* rv = pk ? g.lg_v : g.all_v;
   const int32_t* rjv = pk ? g.lg_j : g.all_j;
   const int64_t nr = pk ? g.nlg : g.nall;
+  // This wave's records are gw, gw + tw, gw + 2 tw, ...: 64 of them at a time
+  // are tested for unfinished sources, one lane each, and only the unfinished
+  // ones are scanned (in the same order), so a late level where most vertices
+  // are finished costs one load chain per 64 records instead of one per record.
   if (!PACKED)
-  for (int64_t r = gw; r < nr; r += tw) {
-    int32_t v = rv[r];
-    u64 sv = b.seen[v];
-    u64 tgt = b.compmask[g.label[v]];
-    if (sv == tgt) continue;
-    int j = rjv[r];
-    const int64_t deg = g.off[v + 1] - g.off[v];
-    int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
-    int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
-    const bool split = deg > g.chunk;
-    const bool fr = fuse && (!split || b.fuse > 1);
-    u64 acc = 0, sacc = 0;
-    double facc = 0.0;
-    for (int64_t eb = e0; eb < e1; eb += 64) {
-      int64_t e = eb + lane;
-      int32_t u = 0;
-      u64 F = 0;
-      if (e < e1) {
-        u = g.adj[e];
-        F = cur[u];
-        acc |= F;
-      }
-      if (!fr) continue;
-      const u64 need = F & ~sv;  // u is a parent of v for these sources
-      u64 P = __ballot(need != 0);
-      while (P) {
-        int q[UNR_BU];
-        bool okq[UNR_BU];
-#pragma unroll
-        for (int t = 0; t < UNR_BU; t++) {
-          okq[t] = P != 0;
-          q[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
-          if (okq[t]) P &= P - 1;
+  for (int64_t k0 = 0; gw + k0 * tw < nr; k0 += 64) {
+    const int64_t rq = gw + (k0 + lane) * tw;
+    int32_t vq = 0, jq = 0;
+    u64 svq = 0;
+    bool open = false;
+    if (rq < nr) {
+      vq = rv[rq];
+      jq = rjv[rq];
+      svq = b.seen[vq];
+      open = svq != b.compmask[g.label[vq]];
+    }
+    u64 O = __ballot(open);
+    while (O) {
+      const int oq = __ffsll((long long)O) - 1;
+      O &= O - 1;
+      const int32_t v = rl32(vq, oq);
+      const u64 sv = rl64(svq, oq);
+      const int j = rl32(jq, oq);
+      const int64_t deg = g.off[v + 1] - g.off[v];
+      int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
+      int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+      const bool split = deg > g.chunk;
+      const bool fr = fuse && (!split || b.fuse > 1);
+      u64 acc = 0, sacc = 0;
+      double facc = 0.0;
+      for (int64_t eb = e0; eb < e1; eb += 64) {
+        int64_t e = eb + lane;
+        int32_t u = 0;
+        u64 F = 0;
+        if (e < e1) {
+          u = g.adj[e];
+          F = cur[u];
+          acc |= F;
         }
-        u64 x[UNR_BU];
-#pragma unroll
-        for (int t = 0; t < UNR_BU; t++) {
-          const int32_t uq = rl32(u, q[t]);
-          const u64 nq = rl64(need, q[t]);
-          const bool use = okq[t] && ((nq >> lane) & 1);
-          x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
-          ca += use;
-        }
-        if (scaled) {
-#pragma unroll
-          for (int t = 0; t < UNR_BU; t++) facc = facc + __longlong_as_double((long long)x[t]);
-        } else {
-#pragma unroll
+        if (!fr) continue;
+        const u64 need = F & ~sv;  // u is a parent of v for these sources
+        u64 P = __ballot(need != 0);
+        while (P) {
+          int q[UNR_BU];
+          bool okq[UNR_BU];
+  #pragma unroll
           for (int t = 0; t < UNR_BU; t++) {
-            u64 s2 = sacc + x[t];
-            if (s2 < sacc) ovf = 1;
-            sacc = s2;
+            okq[t] = P != 0;
+            q[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
+            if (okq[t]) P &= P - 1;
+          }
+          u64 x[UNR_BU];
+  #pragma unroll
+          for (int t = 0; t < UNR_BU; t++) {
+            const int32_t uq = rl32(u, q[t]);
+            const u64 nq = rl64(need, q[t]);
+            const bool use = okq[t] && ((nq >> lane) & 1);
+            x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
+            ca += use;
+          }
+          if (scaled) {
+  #pragma unroll
+            for (int t = 0; t < UNR_BU; t++) facc = facc + __longlong_as_double((long long)x[t]);
+          } else {
+  #pragma unroll
+            for (int t = 0; t < UNR_BU; t++) {
+              u64 s2 = sacc + x[t];
+              if (s2 < sacc) ovf = 1;
+              sacc = s2;
+            }
           }
         }
       }
-    }
-    acc = wave_or64(acc) & ~sv;
-    bool won = false;
-    int64_t dg = 0;
-    if (acc && lane == 0) {
-      if ((nxt[v] & acc) != acc) {
-        u64 old = atomicOr(&nxt[v], acc);
-        won = (old == 0);
-        if (won) {
-          dg = deg;
-          if (g.flags & KF_GATE) lb_set(nlb, v);
+      acc = wave_or64(acc) & ~sv;
+      bool won = false;
+      int64_t dg = 0;
+      if (acc && lane == 0) {
+        if ((nxt[v] & acc) != acc) {
+          u64 old = atomicOr(&nxt[v], acc);
+          won = (old == 0);
+          if (won) {
+            dg = deg;
+            if (g.flags & KF_GATE) lb_set(nlb, v);
+          }
         }
       }
-    }
-    if (fr && ((acc >> lane) & 1)) {
-      u64* row = b.sigma + (int64_t)v * 64 + lane;
-      if (scaled) {
-        const double raw = facc * inv_m;
-        if (!split) {
-          *row = (u64)__double_as_longlong(raw);
-          mx = fmax(mx, raw);
+      if (fr && ((acc >> lane) & 1)) {
+        u64* row = b.sigma + (int64_t)v * 64 + lane;
+        if (scaled) {
+          const double raw = facc * inv_m;
+          if (!split) {
+            *row = (u64)__double_as_longlong(raw);
+            mx = fmax(mx, raw);
+          } else {
+            // every returned sum is a lower bound of a final one; the last is final
+            const double o = atomicAdd((double*)row, raw);
+            mx = fmax(mx, o + raw);
+          }
+        } else if (!split) {
+          if (sg_put(b.sigma, nar, (int64_t)v * 64 + lane, sacc)) ovf = 1;
         } else {
-          // every returned sum is a lower bound of a final one; the last is final
-          const double o = atomicAdd((double*)row, raw);
-          mx = fmax(mx, o + raw);
+          if (sg_add(b.sigma, nar, (int64_t)v * 64 + lane, sacc)) ovf = 1;
         }
-      } else if (!split) {
-        if (sg_put(b.sigma, nar, (int64_t)v * 64 + lane, sacc)) ovf = 1;
-      } else {
-        if (sg_add(b.sigma, nar, (int64_t)v * 64 + lane, sacc)) ovf = 1;
       }
+      app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
     }
-    app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
   }
   if (PACKED) {
     // Vertices of degree <= PACK_DEG, PACK_W per wave: sub-group q (lanes
@@ -1089,97 +1107,117 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
     // sub-group, the parents of each vertex in adjacency order.
     const int sub = lane / PACK_DEG, sl = lane % PACK_DEG;
     const int64_t npk = (g.nsm + PACK_W - 1) / PACK_W;
-    for (int64_t p = gw; p < npk; p += tw) {
-      const int64_t idx = p * PACK_W + sub;
-      int32_t v = 0;
-      u64 sv = 0;
-      int64_t e0 = 0, dgv = 0;
-      if (idx < g.nsm) {
-        v = g.sm_v[idx];
-        sv = b.seen[v];
-        if (sv != b.compmask[g.label[v]]) {
+    // This wave's groups are gw, gw + tw, ... (PACK_W vertices each): the
+    // vertices of 64 / PACK_W groups are tested one lane each, and only the
+    // unfinished ones are scanned, PACK_W at a time in index order.
+    for (int64_t k0 = 0; gw + k0 * tw < npk; k0 += 64 / PACK_W) {
+      const int64_t xq = (gw + (k0 + lane / PACK_W) * tw) * PACK_W + lane % PACK_W;
+      int32_t vq = 0;
+      u64 svq = 0;
+      bool open = false;
+      if (xq < g.nsm) {
+        vq = g.sm_v[xq];
+        svq = b.seen[vq];
+        open = svq != b.compmask[g.label[vq]];
+      }
+      u64 O = __ballot(open);
+      while (O) {
+        // the next PACK_W unfinished vertices, one per sub-group (-1: none)
+        int pick[PACK_W];
+#pragma unroll
+        for (int t = 0; t < PACK_W; t++) {
+          pick[t] = O ? __ffsll((long long)O) - 1 : -1;
+          O &= O - 1;
+        }
+        const int my = sub == 0 ? pick[0] : sub == 1 ? pick[1] : sub == 2 ? pick[2] : pick[3];
+        // both shuffles run on the whole wave (a source lane must be active)
+        const int32_t v = __shfl(vq, my < 0 ? 0 : my, 64);
+        const u64 svs = (u64)__shfl((unsigned long long)svq, my < 0 ? 0 : my, 64);
+        const u64 sv = my < 0 ? 0ull : svs;
+        int64_t e0 = 0, dgv = 0;
+        if (my >= 0) {
           e0 = g.off[v];
           dgv = g.off[v + 1] - e0;
         }
-      }
-      int32_t u = 0;
-      u64 need = 0;
-      if (sl < dgv) {
-        u = g.adj[e0 + sl];
-        need = cur[u] & ~sv;  // u is a parent of v for these sources
-      }
-      u64 acc = need;
-#pragma unroll
-      for (int o = PACK_DEG / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
-      u64 sq[PACK_W];  // per sub-group: exact sum, or FP64 bits when scaled
-#pragma unroll
-      for (int q = 0; q < PACK_W; q++) sq[q] = 0;
-      if (fuse) {
-        u64 P = __ballot(need != 0);
-        while (P) {
-          int qa[UNR_BU];
-          bool okq[UNR_BU];
-#pragma unroll
-          for (int t = 0; t < UNR_BU; t++) {
-            okq[t] = P != 0;
-            qa[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
-            if (okq[t]) P &= P - 1;
-          }
-          u64 x[UNR_BU];
-#pragma unroll
-          for (int t = 0; t < UNR_BU; t++) {
-            const int32_t uq = rl32(u, qa[t]);
-            const u64 nq = rl64(need, qa[t]);
-            const bool use = okq[t] && ((nq >> lane) & 1);
-            x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
-            ca += use;
-          }
-#pragma unroll
-          for (int t = 0; t < UNR_BU; t++) {
-            const int gq = qa[t] / PACK_DEG;  // wave-uniform
-#pragma unroll
-            for (int q = 0; q < PACK_W; q++) {
-              if (gq != q || !okq[t]) continue;
-              if (scaled) {
-                sq[q] = (u64)__double_as_longlong(__longlong_as_double((long long)sq[q]) +
-                                                  __longlong_as_double((long long)x[t]));
-              } else {
-                const u64 s2 = sq[q] + x[t];
-                if (s2 < sq[q]) ovf = 1;
-                sq[q] = s2;
+        int32_t u = 0;
+        u64 need = 0;
+        if (sl < dgv) {
+          u = g.adj[e0 + sl];
+          need = cur[u] & ~sv;  // u is a parent of v for these sources
+        }
+        u64 acc = need;
+  #pragma unroll
+        for (int o = PACK_DEG / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
+        u64 sq[PACK_W];  // per sub-group: exact sum, or FP64 bits when scaled
+  #pragma unroll
+        for (int q = 0; q < PACK_W; q++) sq[q] = 0;
+        if (fuse) {
+          u64 P = __ballot(need != 0);
+          while (P) {
+            int qa[UNR_BU];
+            bool okq[UNR_BU];
+  #pragma unroll
+            for (int t = 0; t < UNR_BU; t++) {
+              okq[t] = P != 0;
+              qa[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
+              if (okq[t]) P &= P - 1;
+            }
+            u64 x[UNR_BU];
+  #pragma unroll
+            for (int t = 0; t < UNR_BU; t++) {
+              const int32_t uq = rl32(u, qa[t]);
+              const u64 nq = rl64(need, qa[t]);
+              const bool use = okq[t] && ((nq >> lane) & 1);
+              x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
+              ca += use;
+            }
+  #pragma unroll
+            for (int t = 0; t < UNR_BU; t++) {
+              const int gq = qa[t] / PACK_DEG;  // wave-uniform
+  #pragma unroll
+              for (int q = 0; q < PACK_W; q++) {
+                if (gq != q || !okq[t]) continue;
+                if (scaled) {
+                  sq[q] = (u64)__double_as_longlong(__longlong_as_double((long long)sq[q]) +
+                                                    __longlong_as_double((long long)x[t]));
+                } else {
+                  const u64 s2 = sq[q] + x[t];
+                  if (s2 < sq[q]) ovf = 1;
+                  sq[q] = s2;
+                }
               }
             }
           }
         }
-      }
-      bool won = false;
-      int64_t dg = 0;
-      if (sl == 0 && acc) {
-        if ((nxt[v] & acc) != acc) {
-          u64 old = atomicOr(&nxt[v], acc);
-          won = (old == 0);
-          if (won) {
-            dg = dgv;
-            if (g.flags & KF_GATE) lb_set(nlb, v);
+        bool won = false;
+        int64_t dg = 0;
+        if (sl == 0 && acc) {
+          if ((nxt[v] & acc) != acc) {
+            u64 old = atomicOr(&nxt[v], acc);
+            won = (old == 0);
+            if (won) {
+              dg = dgv;
+              if (g.flags & KF_GATE) lb_set(nlb, v);
+            }
           }
         }
-      }
-      if (fuse) {
-#pragma unroll
-        for (int q = 0; q < PACK_W; q++) {
-          const u64 aq = rl64(acc, q * PACK_DEG);
-          if (!((aq >> lane) & 1)) continue;
-          const int64_t row = (int64_t)rl32(v, q * PACK_DEG) * 64 + lane;
-          if (scaled) {
-            const double raw = __longlong_as_double((long long)sq[q]) * inv_m;
-            b.sigma[row] = (u64)__double_as_longlong(raw);
-            mx = fmax(mx, raw);
-          } else if (sg_put(b.sigma, nar, row, sq[q])) {
-            ovf = 1;
+        if (fuse) {
+  #pragma unroll
+          for (int q = 0; q < PACK_W; q++) {
+            const u64 aq = rl64(acc, q * PACK_DEG);
+            if (!((aq >> lane) & 1)) continue;
+            const int64_t row = (int64_t)rl32(v, q * PACK_DEG) * 64 + lane;
+            if (scaled) {
+              const double raw = __longlong_as_double((long long)sq[q]) * inv_m;
+              b.sigma[row] = (u64)__double_as_longlong(raw);
+              mx = fmax(mx, raw);
+            } else if (sg_put(b.sigma, nar, row, sq[q])) {
+              ovf = 1;
+            }
           }
         }
+        app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
       }
-      app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
     }
   }
   app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
