@@ -625,41 +625,58 @@ __device__ __forceinline__ void k_plain_bu_body(GA g, PA p) {
   long long acnt = 0;
   int ovf = 0;
   APPENDER(ap);
-  for (int64_t r = gw; r < g.nall; r += tw) {
-    int32_t v = g.all_v[r];
-    if (g.label[v] != lab) continue;
-    int dv = p.depth[v];
-    if (dv != -1 && dv != d + 1) continue;
-    int j = g.all_j[r];
-    int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
-    int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
-    u64 acc = 0;
-    int cnt = 0;
-    for (int64_t eb = e0; eb < e1; eb += 64) {
-      int64_t e = eb + lane;
-      if (e < e1) {
-        int32_t u = adj_at(g, g.adj, e);
-        if (p.depth[u] == d) {
-          u64 su = p.sigma[u];
-          u64 s = acc + su;
-          if (s < acc) ovf = 1;
-          acc = s;
-          cnt++;
-        }
+  // This wave's records are gw, gw + tw, ...: 64 at a time are tested (one
+  // lane each) for a vertex of the source's component that is unvisited or
+  // on level d+1 (depths only move from -1 to d+1 during this kernel), and
+  // only those are scanned, in the same order.
+  for (int64_t k0 = 0; gw + k0 * tw < g.nall; k0 += 64) {
+    const int64_t rq = gw + (k0 + lane) * tw;
+    int32_t vq = 0, jq = 0;
+    bool open = false;
+    if (rq < g.nall) {
+      vq = g.all_v[rq];
+      jq = g.all_j[rq];
+      if (g.label[vq] == lab) {
+        const int dv = p.depth[vq];
+        open = dv == -1 || dv == d + 1;
       }
     }
-    acc = wave_sum_u64_ovf(acc, &ovf);
-    cnt = (int)wave_sum_ll(cnt);
-    bool won = false;
-    if (cnt > 0 && lane == 0) {
-      int old = atomicCAS(&p.depth[v], -1, d + 1);
-      won = (old == -1);
-      u64 o = atomicAdd(&p.sigma[v], acc);
-      if (o + acc < o) ovf = 1;
-      if (won) p.delta[v] = g.w ? (double)g.w[v] : 0.0;
-      acnt += cnt;
+    u64 O = __ballot(open);
+    while (O) {
+      const int oq = __ffsll((long long)O) - 1;
+      O &= O - 1;
+      const int32_t v = rl32(vq, oq);
+      const int j = rl32(jq, oq);
+      int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
+      int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+      u64 acc = 0;
+      int cnt = 0;
+      for (int64_t eb = e0; eb < e1; eb += 64) {
+        int64_t e = eb + lane;
+        if (e < e1) {
+          int32_t u = adj_at(g, g.adj, e);
+          if (p.depth[u] == d) {
+            u64 su = p.sigma[u];
+            u64 s = acc + su;
+            if (s < acc) ovf = 1;
+            acc = s;
+            cnt++;
+          }
+        }
+      }
+      acc = wave_sum_u64_ovf(acc, &ovf);
+      cnt = (int)wave_sum_ll(cnt);
+      bool won = false;
+      if (cnt > 0 && lane == 0) {
+        int old = atomicCAS(&p.depth[v], -1, d + 1);
+        won = (old == -1);
+        u64 o = atomicAdd(&p.sigma[v], acc);
+        if (o + acc < o) ovf = 1;
+        if (won) p.delta[v] = g.w ? (double)g.w[v] : 0.0;
+        acnt += cnt;
+      }
+      app_push(ap, won, v, won ? g.off[v + 1] - g.off[v] : 0, g.chunk, c, p.rec_v, p.rec_j, p.cap);
     }
-    app_push(ap, won, v, won ? g.off[v + 1] - g.off[v] : 0, g.chunk, c, p.rec_v, p.rec_j, p.cap);
   }
   app_finish(ap, c, p.rec_v, p.rec_j, p.cap);
   acnt = wave_sum_ll(acnt);
@@ -856,24 +873,27 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled, int narrow) {
     }
   }
   __syncthreads();
-  if (i == 0) {
-    // Degree sum of the components holding sources, and of the sources
-    // already complete (reached by every source of their component).
-    int64_t reach = 0, done = 0;
-    for (int q = 0; q < k; q++) {
-      int32_t sq = b.srcbuf[q];
-      int lq = g.label[sq];
-      bool first = true;
-      for (int t = 0; t < q; t++)
-        if (g.label[b.srcbuf[t]] == lq) { first = false; break; }
-      if (first) reach += g.comp_arcs[lq];
-      bool firstv = true;
-      for (int t = 0; t < q; t++)
-        if (b.srcbuf[t] == sq) { firstv = false; break; }
-      if (firstv && b.seen[sq] == b.compmask[lq]) done += g.off[sq + 1] - g.off[sq];
+  // Degree sum of the components holding sources, and of the sources already
+  // complete (reached by every source of their component): lane i counts its
+  // source's component and vertex if no lower lane holds the same one.
+  {
+    const int32_t lq = i < k ? g.label[s] : -1;
+    bool first = i < k, firstv = i < k;
+    for (int t = 0; t < 64; t++) {
+      const int32_t lt = __shfl(lq, t, 64), st = __shfl(s, t, 64);
+      if (t < i) {
+        if (lt == lq) first = false;
+        if (st == s) firstv = false;
+      }
     }
-    c->arcs_reach = reach;
-    c->arcs_seen = done;
+    long long reach = first ? (long long)g.comp_arcs[lq] : 0;
+    long long done = (firstv && b.seen[s] == b.compmask[lq]) ? (long long)(g.off[s + 1] - g.off[s]) : 0;
+    reach = wave_sum_ll(reach);
+    done = wave_sum_ll(done);
+    if (i == 0) {
+      c->arcs_reach = reach;
+      c->arcs_seen = done;
+    }
   }
 }
 
