@@ -986,6 +986,7 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 // the records of a split vertex (fuse_bu 2) add theirs atomically into its
 // row, whose columns are zero until the source reaches it.
 #define UNR_BU 8
+#define UNR_PK 4  // parent gathers in flight per wave in the packed bottom-up kernel
 #define PACK_DEG 16  // packed bottom-up: vertices of degree <= PACK_DEG ...
 #define PACK_W 4     // ... PACK_W to a wave (PACK_DEG * PACK_W = 64)
 static_assert(PACK_DEG * PACK_W == 64 && PACK_W == 4, "the packed bottom-up scan picks one vertex per sub-group of 16 lanes");
@@ -1174,17 +1175,17 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         if (fuse) {
           u64 P = __ballot(need != 0);
           while (P) {
-            int qa[UNR_BU];
-            bool okq[UNR_BU];
+            int qa[UNR_PK];
+            bool okq[UNR_PK];
   #pragma unroll
-            for (int t = 0; t < UNR_BU; t++) {
+            for (int t = 0; t < UNR_PK; t++) {
               okq[t] = P != 0;
               qa[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
               if (okq[t]) P &= P - 1;
             }
-            u64 x[UNR_BU];
+            u64 x[UNR_PK];
   #pragma unroll
-            for (int t = 0; t < UNR_BU; t++) {
+            for (int t = 0; t < UNR_PK; t++) {
               const int32_t uq = rl32(u, qa[t]);
               const u64 nq = rl64(need, qa[t]);
               const bool use = okq[t] && ((nq >> lane) & 1);
@@ -1192,7 +1193,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
               ca += use;
             }
   #pragma unroll
-            for (int t = 0; t < UNR_BU; t++) {
+            for (int t = 0; t < UNR_PK; t++) {
               const int gq = qa[t] / PACK_DEG;  // wave-uniform
   #pragma unroll
               for (int q = 0; q < PACK_W; q++) {
@@ -1256,7 +1257,7 @@ __global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
 }
 
 // The packed low-degree vertices, in a kernel of their own (own register budget).
-__global__ void __launch_bounds__(256) k_ms_bu_pk(GA g, BA b, KT* kt) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ms_bu_pk(GA g, BA b, KT* kt) {
   kt_enter(kt);
   if (b.ctl->narrow) k_ms_bu_body<true, true>(g, b);
   else k_ms_bu_body<false, true>(g, b);
