@@ -1059,14 +1059,14 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         while (P) {
           int q[UNR_BU];
           bool okq[UNR_BU];
-  #pragma unroll
+#pragma unroll
           for (int t = 0; t < UNR_BU; t++) {
             okq[t] = P != 0;
             q[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
             if (okq[t]) P &= P - 1;
           }
           u64 x[UNR_BU];
-  #pragma unroll
+#pragma unroll
           for (int t = 0; t < UNR_BU; t++) {
             const int32_t uq = rl32(u, q[t]);
             const u64 nq = rl64(need, q[t]);
@@ -1075,10 +1075,10 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
             ca += use;
           }
           if (scaled) {
-  #pragma unroll
+#pragma unroll
             for (int t = 0; t < UNR_BU; t++) facc = facc + __longlong_as_double((long long)x[t]);
           } else {
-  #pragma unroll
+#pragma unroll
             for (int t = 0; t < UNR_BU; t++) {
               u64 s2 = sacc + x[t];
               if (s2 < sacc) ovf = 1;
@@ -1167,24 +1167,24 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
           need = cur[u] & ~sv;  // u is a parent of v for these sources
         }
         u64 acc = need;
-  #pragma unroll
+#pragma unroll
         for (int o = PACK_DEG / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
         u64 sq[PACK_W];  // per sub-group: exact sum, or FP64 bits when scaled
-  #pragma unroll
+#pragma unroll
         for (int q = 0; q < PACK_W; q++) sq[q] = 0;
         if (fuse) {
           u64 P = __ballot(need != 0);
           while (P) {
             int qa[UNR_PK];
             bool okq[UNR_PK];
-  #pragma unroll
+#pragma unroll
             for (int t = 0; t < UNR_PK; t++) {
               okq[t] = P != 0;
               qa[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
               if (okq[t]) P &= P - 1;
             }
             u64 x[UNR_PK];
-  #pragma unroll
+#pragma unroll
             for (int t = 0; t < UNR_PK; t++) {
               const int32_t uq = rl32(u, qa[t]);
               const u64 nq = rl64(need, qa[t]);
@@ -1192,10 +1192,10 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
               x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
               ca += use;
             }
-  #pragma unroll
+#pragma unroll
             for (int t = 0; t < UNR_PK; t++) {
               const int gq = qa[t] / PACK_DEG;  // wave-uniform
-  #pragma unroll
+#pragma unroll
               for (int q = 0; q < PACK_W; q++) {
                 if (gq != q || !okq[t]) continue;
                 if (scaled) {
@@ -1223,7 +1223,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
           }
         }
         if (fuse) {
-  #pragma unroll
+#pragma unroll
           for (int q = 0; q < PACK_W; q++) {
             const u64 aq = rl64(acc, q * PACK_DEG);
             if (!((aq >> lane) & 1)) continue;
