@@ -1535,17 +1535,14 @@ __global__ void __launch_bounds__(256) k_ms_push(GA g, BA b) {
 // level d+2 from lm[(d+2)%3].
 // Backward level d, one record (the pull): see k_ms_back_body.
 template <bool NAR>
-__device__ __forceinline__ void back_one(const GA& g, const BA& b, int64_t r, int d, const u64* lmn, u64* lmd,
-                                         const unsigned* nlb, unsigned* dlb, double mult, bool scaled,
-                                         double inv_md) {
+__device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, int j, u64 L, int64_t o0, int64_t o1,
+                                         int d, const u64* lmn, u64* lmd, const unsigned* nlb, unsigned* dlb,
+                                         double mult, bool scaled, double inv_md) {
   const int lane = lane_id();
-  int32_t v = b.rec_v[r];
-  int j = b.rec_j[r];
-  u64 L = b.rec_mask[r - j];
-  int64_t deg = g.off[v + 1] - g.off[v];
+  int64_t deg = o1 - o0;
   int nch = nchunks(deg, g.chunk);
-  int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
-  int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+  int64_t e0 = o0 + (int64_t)j * g.chunk;
+  int64_t e1 = min(o1, e0 + g.chunk);
   const bool mine = (L >> lane) & 1;
   const double wv = g.w ? (double)g.w[v] : 0.0;
   const u64 sbits = mine ? sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane) : 1ull;
@@ -1636,7 +1633,28 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
       }
     }
   } else {
-    for (int64_t r = lo + gw; r < hi; r += tw) back_one<NAR>(g, b, r, d, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
+    // This wave's records lo + gw, lo + gw + tw, ...: the headers of 64 of
+    // them load at once, one lane each, then the records run in order.
+    for (int64_t k0 = 0; lo + gw + k0 * tw < hi; k0 += 64) {
+      const int64_t rq = lo + gw + (k0 + lane) * tw;
+      int32_t vq = 0, jq = 0;
+      u64 Lq = 0;
+      int64_t o0q = 0, o1q = 0;
+      if (rq < hi) {
+        vq = b.rec_v[rq];
+        jq = b.rec_j[rq];
+        Lq = b.rec_mask[rq - jq];
+        o0q = g.off[vq];
+        o1q = g.off[vq + 1];
+      }
+      u64 R = __ballot(rq < hi);
+      while (R) {
+        const int q = __ffsll((long long)R) - 1;
+        R &= R - 1;
+        back_one<NAR>(g, b, rl32(vq, q), rl32(jq, q), rl64(Lq, q), (int64_t)rl64((u64)o0q, q),
+                      (int64_t)rl64((u64)o1q, q), d, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
+      }
+    }
   }
   {
     u64* old = b.lm + (int64_t)((d + 2) % 3) * b.n;
