@@ -1023,11 +1023,14 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
     const int64_t rq = gw + (k0 + lane) * tw;
     int32_t vq = 0, jq = 0;
     u64 svq = 0;
+    int64_t o0q = 0, o1q = 0;
     bool open = false;
     if (rq < nr) {
       vq = rv[rq];
       jq = rjv[rq];
       svq = b.seen[vq];
+      o0q = g.off[vq];
+      o1q = g.off[vq + 1];
       open = svq != b.compmask[g.label[vq]];
     }
     u64 O = __ballot(open);
@@ -1037,9 +1040,10 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       const int32_t v = rl32(vq, oq);
       const u64 sv = rl64(svq, oq);
       const int j = rl32(jq, oq);
-      const int64_t deg = g.off[v + 1] - g.off[v];
-      int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
-      int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
+      const int64_t o0 = (int64_t)rl64((u64)o0q, oq), o1 = (int64_t)rl64((u64)o1q, oq);
+      const int64_t deg = o1 - o0;
+      int64_t e0 = o0 + (int64_t)j * g.chunk;
+      int64_t e1 = min(o1, e0 + g.chunk);
       const bool split = deg > g.chunk;
       const bool fr = fuse && (!split || b.fuse > 1);
       u64 acc = 0, sacc = 0;
@@ -1135,10 +1139,13 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       const int64_t xq = (gw + (k0 + lane / PACK_W) * tw) * PACK_W + lane % PACK_W;
       int32_t vq = 0;
       u64 svq = 0;
+      int64_t o0q = 0, o1q = 0;
       bool open = false;
       if (xq < g.nsm) {
         vq = g.sm_v[xq];
         svq = b.seen[vq];
+        o0q = g.off[vq];
+        o1q = g.off[vq + 1];
         open = svq != b.compmask[g.label[vq]];
       }
       u64 O = __ballot(open);
@@ -1155,11 +1162,9 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         const int32_t v = __shfl(vq, my < 0 ? 0 : my, 64);
         const u64 svs = (u64)__shfl((unsigned long long)svq, my < 0 ? 0 : my, 64);
         const u64 sv = my < 0 ? 0ull : svs;
-        int64_t e0 = 0, dgv = 0;
-        if (my >= 0) {
-          e0 = g.off[v];
-          dgv = g.off[v + 1] - e0;
-        }
+        const int64_t o0s = (int64_t)__shfl((unsigned long long)o0q, my < 0 ? 0 : my, 64);
+        const int64_t o1s = (int64_t)__shfl((unsigned long long)o1q, my < 0 ? 0 : my, 64);
+        const int64_t e0 = my < 0 ? 0 : o0s, dgv = my < 0 ? 0 : o1s - o0s;
         int32_t u = 0;
         u64 need = 0;
         if (sl < dgv) {
