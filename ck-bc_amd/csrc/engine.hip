@@ -1675,7 +1675,7 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_ms_back(GA g, BA b, KT* kt) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_ms_back(GA g, BA b, KT* kt) {
   kt_enter(kt);
   if (b.ctl->narrow) k_ms_back_body<true>(g, b);
   else k_ms_back_body<false>(g, b);
