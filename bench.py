@@ -321,6 +321,7 @@ def main():
         "plain_gteps": round(gteps_plain, 3) if gteps_plain else None,
         "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain else None,
         "model_bytes_gbs": round(st["bytes"] / t_batched / 1e9, 1),
+        "levels_per_vertex_batch": round(st["sum_pairs"] / max(1, st["sum_union"]), 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
