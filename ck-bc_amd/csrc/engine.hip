@@ -568,36 +568,53 @@ __device__ __forceinline__ void k_plain_td_body(GA g, PA p) {
   long long acnt = 0;
   int ovf = 0;
   APPENDER(ap);
-  for (int64_t r = lo + gw; r < hi; r += tw) {
-    int32_t u = p.rec_v[r];
-    int j = p.rec_j[r];
-    int64_t e0 = g.off[u] + (int64_t)j * g.chunk;
-    int64_t e1 = min(g.off[u + 1], e0 + g.chunk);
-    u64 su = p.sigma[u];
-    for (int64_t eb = e0; eb < e1; eb += 64) {
-      int64_t e = eb + lane;
-      bool valid = e < e1;
-      int32_t v = valid ? adj_at(g, g.adj, e) : 0;
-      bool won = false;
-      int64_t dg = 0;
-      if (valid) {
-        int dv = p.depth[v];
-        if (dv == -1) {
-          int old = atomicCAS(&p.depth[v], -1, d + 1);
-          won = (old == -1);
-          dv = won ? d + 1 : old;
+  // This wave's records lo + gw, lo + gw + tw, ...: the headers of 64 of them
+  // load at once, one lane each, then the records run in order.
+  for (int64_t k0 = 0; lo + gw + k0 * tw < hi; k0 += 64) {
+    const int64_t rq = lo + gw + (k0 + lane) * tw;
+    int32_t uq = 0, jq = 0;
+    int64_t o0q = 0, o1q = 0;
+    u64 suq = 0;
+    if (rq < hi) {
+      uq = p.rec_v[rq];
+      jq = p.rec_j[rq];
+      o0q = g.off[uq];
+      o1q = g.off[uq + 1];
+      suq = p.sigma[uq];
+    }
+    u64 R = __ballot(rq < hi);
+    while (R) {
+      const int q = __ffsll((long long)R) - 1;
+      R &= R - 1;
+      const int j = rl32(jq, q);
+      const u64 su = rl64(suq, q);
+      const int64_t e0 = (int64_t)rl64((u64)o0q, q) + (int64_t)j * g.chunk;
+      const int64_t e1 = min((int64_t)rl64((u64)o1q, q), e0 + g.chunk);
+      for (int64_t eb = e0; eb < e1; eb += 64) {
+        int64_t e = eb + lane;
+        bool valid = e < e1;
+        int32_t v = valid ? adj_at(g, g.adj, e) : 0;
+        bool won = false;
+        int64_t dg = 0;
+        if (valid) {
+          int dv = p.depth[v];
+          if (dv == -1) {
+            int old = atomicCAS(&p.depth[v], -1, d + 1);
+            won = (old == -1);
+            dv = won ? d + 1 : old;
+          }
+          if (dv == d + 1) {
+            u64 o = atomicAdd(&p.sigma[v], su);
+            if (o + su < o) ovf = 1;
+            acnt++;
+          }
+          if (won) {
+            p.delta[v] = g.w ? (double)g.w[v] : 0.0;
+            dg = g.off[v + 1] - g.off[v];
+          }
         }
-        if (dv == d + 1) {
-          u64 o = atomicAdd(&p.sigma[v], su);
-          if (o + su < o) ovf = 1;
-          acnt++;
-        }
-        if (won) {
-          p.delta[v] = g.w ? (double)g.w[v] : 0.0;
-          dg = g.off[v + 1] - g.off[v];
-        }
+        app_push(ap, won, v, dg, g.chunk, c, p.rec_v, p.rec_j, p.cap);
       }
-      app_push(ap, won, v, dg, g.chunk, c, p.rec_v, p.rec_j, p.cap);
     }
   }
   app_finish(ap, c, p.rec_v, p.rec_j, p.cap);
@@ -744,31 +761,48 @@ __device__ __forceinline__ void k_plain_back_body(GA g, PA p) {
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   const double mult = p.multbuf[0];
-  for (int64_t r = lo + gw; r < hi; r += tw) {
-    int32_t v = p.rec_v[r];
-    int j = p.rec_j[r];
-    int64_t deg = g.off[v + 1] - g.off[v];
-    int nch = nchunks(deg, g.chunk);
-    int64_t e0 = g.off[v] + (int64_t)j * g.chunk;
-    int64_t e1 = min(g.off[v + 1], e0 + g.chunk);
-    double sv = (double)p.sigma[v];
-    double acc = 0.0;
-    for (int64_t eb = e0; eb < e1; eb += 64) {
-      int64_t e = eb + lane;
-      if (e < e1) {
-        int32_t z = adj_at(g, g.adj, e);
-        if (p.depth[z] == d + 1) acc = acc + p.coef[z] * sv;
-      }
+  // headers of 64 of this wave's records at once, as in k_plain_td
+  for (int64_t k0 = 0; lo + gw + k0 * tw < hi; k0 += 64) {
+    const int64_t rq = lo + gw + (k0 + lane) * tw;
+    int32_t vq = 0, jq = 0;
+    int64_t o0q = 0, o1q = 0;
+    u64 sq = 0;
+    if (rq < hi) {
+      vq = p.rec_v[rq];
+      jq = p.rec_j[rq];
+      o0q = g.off[vq];
+      o1q = g.off[vq + 1];
+      sq = p.sigma[vq];
     }
-    acc = wave_sum_d(acc);
-    if (lane == 0) {
-      if (nch == 1) {
-        double dl = p.delta[v] + acc;
-        p.delta[v] = dl;
-        p.coef[v] = (dl + 1.0) / sv;
-        p.bc[v] += mult * dl;
-      } else {
-        atomicAdd(&p.delta[v], acc);  // finished by k_plain_bfin
+    u64 R = __ballot(rq < hi);
+    while (R) {
+      const int q = __ffsll((long long)R) - 1;
+      R &= R - 1;
+      const int32_t v = rl32(vq, q);
+      const int j = rl32(jq, q);
+      const int64_t o0 = (int64_t)rl64((u64)o0q, q), o1 = (int64_t)rl64((u64)o1q, q);
+      const int nch = nchunks(o1 - o0, g.chunk);
+      const int64_t e0 = o0 + (int64_t)j * g.chunk;
+      const int64_t e1 = min(o1, e0 + g.chunk);
+      const double sv = (double)rl64(sq, q);
+      double acc = 0.0;
+      for (int64_t eb = e0; eb < e1; eb += 64) {
+        int64_t e = eb + lane;
+        if (e < e1) {
+          int32_t z = adj_at(g, g.adj, e);
+          if (p.depth[z] == d + 1) acc = acc + p.coef[z] * sv;
+        }
+      }
+      acc = wave_sum_d(acc);
+      if (lane == 0) {
+        if (nch == 1) {
+          double dl = p.delta[v] + acc;
+          p.delta[v] = dl;
+          p.coef[v] = (dl + 1.0) / sv;
+          p.bc[v] += mult * dl;
+        } else {
+          atomicAdd(&p.delta[v], acc);  // finished by k_plain_bfin
+        }
       }
     }
   }
