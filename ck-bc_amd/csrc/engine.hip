@@ -1295,7 +1295,8 @@ __global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
   kt_exit(kt);
 }
 
-// The packed low-degree vertices, in a kernel of their own (own register budget).
+// The packed low-degree vertices, in a kernel of their own (own register budget:
+// 5 waves per SIMD, 96 VGPRs; its load chains need the extra resident wave).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ms_bu_pk(GA g, BA b, KT* kt) {
   kt_enter(kt);
   if (b.ctl->narrow) k_ms_bu_body<true, true>(g, b);
@@ -1709,6 +1710,8 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   }
 }
 
+// 8 waves per SIMD (64 VGPRs; a few bytes spill outside the gather loop): more
+// row gathers in flight per CU than 7 waves with no spill.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_ms_back(GA g, BA b, KT* kt) {
   kt_enter(kt);
   if (b.ctl->narrow) k_ms_back_body<true>(g, b);
