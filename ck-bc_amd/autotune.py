@@ -27,9 +27,9 @@ import bcmi  # noqa: E402
 SPACE = {
     "block": [64, 128, 256],
     "grid": [1024, 2048, 4096, 8192, 16384],
-    "grid_back": [0, 4096, 8192, 16384],
-    "alpha_batched": [250, 500, 1000, 2000, 4000, 1 << 60],
-    "beta_batched": [4, 8, 16, 32, 64],
+    "grid_back": [0, 4096, 8192, 16384, 32768],
+    "alpha_batched": [125, 250, 500, 1000, 2000, 4000, 1 << 60],
+    "beta_batched": [4, 8, 16, 32, 64, 128, 512],
     "gamma_push": [0, 4, 8, 16, 32],
     "fuse_bu": [1, 2],
     "pack_bu": [0, 1],
