@@ -193,3 +193,31 @@ def test_ck_dataset_entries_and_generator(tmp_path):
                            "--edgefactor", "16", "--seed", "20250101", "--out", str(txt), "--text"])
     t = bcmi.Graph.read_edgelist(str(txt), g.n)
     assert np.array_equal(t.off, g.off) and np.array_equal(t.adj, g.adj)
+
+
+def test_tuned_table_names_known_parameters():
+    """Every entry of the autotuned table only holds parameters the tuner
+    applies (plus its own notes), with values the engine accepts, and apply()
+    sets exactly those on an engine."""
+    import autotune
+    table = autotune.load()
+    assert {"default", "rmat-s18-ef16", "rmat-s22-ef16", "rmat-s25-ef16"} <= set(table)
+    notes = {"batched_gteps", "plain_gteps", "tuned_on"}
+    applied = {"chunk", "block", "grid", "grid_back", "grid_plain", "alpha_batched", "beta_batched", "gamma_push",
+               "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width"}
+
+    class Rec:
+        def __init__(self):
+            self.set = {}
+
+        def set_param(self, k, v):
+            self.set[k] = v
+
+    for key, entry in table.items():
+        assert set(entry) <= applied | notes, (key, set(entry) - applied - notes)
+        assert entry["block"] in (64, 128, 256) and entry["chunk"] >= 64 and 1 <= entry["width"] <= 64
+        assert entry["steps"] >= 1 and entry["grid"] >= 1
+        r = Rec()
+        autotune.apply(r, table, key)
+        assert r.set == {k: int(v) for k, v in entry.items() if k in applied}
+    assert autotune.graph_key("rmat", scale=22, edgefactor=16) == "rmat-s22-ef16"
