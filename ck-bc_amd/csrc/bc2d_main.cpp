@@ -10,7 +10,8 @@
 // -F file (sources, one id per line), -P (full precision %.17g in the result
 // file), -X (with -H 1: exact BC over the listed sources instead of the
 // reference's "a core source stands for its pruned tree" semantics),
-// -S/-E generate an R-MAT graph (disabled in the reference, bc2d.c:664).
+// -S/-E generate an R-MAT graph (disabled in the reference, bc2d.c:664),
+// -T key=value sets an engine tunable (repeatable; bcmi_set_param).
 // Errors print a message and exit 1 like prexit (bc2d.c:64-72).
 #include <getopt.h>
 #include <sys/stat.h>
@@ -51,6 +52,7 @@ static void usage(const char* p) {
       "\t\t -X                 with -H 1: exact BC over exactly the listed sources\n"
       "\t\t -C <file>          binary CSR cache of the -f graph: used when it matches\n"
       "\t\t                    the text file's size and time, (re)written otherwise\n"
+      "\t\t -T <key>=<value>   engine tunable (integer; repeatable), e.g. -T grid=2048\n"
       "\n",
       p);
 }
@@ -71,11 +73,12 @@ int main(int argc, char** argv) {
   uint64_t N = 0, startv = 0, nbfs = 1, seed = 0;
   int have_seed = 0, full_prec = 0, exact = 0, mode = BCMI_MODE_BATCHED;
   std::string gfile, ofile, sfile, cfile;
+  std::vector<std::pair<std::string, long long>> tunables;  // -T key=value, applied before the upload
   printf("MPI Tasks 1\n");
   for (int i = 0; i < argc; i++) printf(" %s", argv[i]);
   printf("\n\n");
   int opt;
-  while ((opt = getopt(argc, argv, "o:p:amhDR:Uf:n:r:S:E:N:H:B:s:F:PXC:")) != -1) {
+  while ((opt = getopt(argc, argv, "o:p:amhDR:Uf:n:r:S:E:N:H:B:s:F:PXC:T:")) != -1) {
     switch (opt) {
       case 'H':
         if (sscanf(optarg, "%d", &heuristic) != 1) die("Invalid Heuristic Option (-H): %s\n", optarg);
@@ -132,6 +135,15 @@ int main(int argc, char** argv) {
       case 'P': full_prec = 1; break;
       case 'X': exact = 1; break;
       case 'C': cfile = optarg; break;
+      case 'T': {
+        const char* eq = strchr(optarg, '=');
+        char* end = nullptr;
+        long long v = eq ? strtoll(eq + 1, &end, 10) : 0;
+        if (!eq || eq == optarg || !end || end == eq + 1 || *end)
+          die("Invalid tunable (-T key=value with an integer value): %s\n", optarg);
+        tunables.emplace_back(std::string(optarg, eq - optarg), v);
+        break;
+      }
       default: usage(argv[0]);
     }
   }
@@ -220,6 +232,7 @@ int main(int argc, char** argv) {
 
   bcmi_ctx* ctx = nullptr;
   CK(bcmi_create(0, &ctx));
+  for (const auto& t : tunables) CK(bcmi_set_param(ctx, t.first.c_str(), (int64_t)t.second));
   t0 = now_s();
   CK(bcmi_upload_graph(ctx, g->n, g->off, g->adj, g->directed, heuristic == 1 ? BCMI_H_PEEL : BCMI_H_NONE));
   double redu_time = heuristic == 1 ? now_s() - t0 : 0.0;

@@ -57,6 +57,26 @@ def test_ck_run_heuristic_full_precision(tmp_path):
     assert np.max(np.abs(got[nz] - want[nz]) / want[nz]) < 1e-9
 
 
+def test_ck_run_tunables(tmp_path):
+    """-T key=value reaches the engine: a launch shape of small records, few
+    levels per captured graph and early bottom-up gives the same BC; an
+    unknown key fails with exit status 1."""
+    fn, f = write_facebook(tmp_path)
+    out = tmp_path / "res"
+    r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "4039", "-N", "1000", "-P", "-o", str(out),
+                        "-T", "chunk=64", "-T", "steps=4", "-T", "alpha_batched=125", "-T", "grid=2048"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = read_result(str(out) + "_1X1_0.log")
+    want = f["bc_src1000"]
+    nz = want != 0
+    assert np.all(got[~nz] == 0)
+    assert np.max(np.abs(got[nz] - want[nz]) / want[nz]) < 1e-9
+    r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "4039", "-N", "10", "-T", "nosuch=1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "nosuch" in (r.stdout + r.stderr)
+
+
 def test_ck_run_errors(tmp_path):
     fn, _ = write_facebook(tmp_path)
     r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "100"], capture_output=True, text=True, timeout=120)

@@ -221,3 +221,14 @@ def test_tuned_table_names_known_parameters():
         autotune.apply(r, table, key)
         assert r.set == {k: int(v) for k, v in entry.items() if k in applied}
     assert autotune.graph_key("rmat", scale=22, edgefactor=16) == "rmat-s22-ef16"
+
+
+def test_cli_rejects_malformed_tunables():
+    """-T needs key=value with an integer value; the check comes before any
+    device call, so it runs without a GPU."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ck-bc_amd", "ck-bc-hip")
+    for bad in ("grid", "=4", "grid=", "grid=12a"):
+        r = subprocess.run([exe, "-p", "1x1", "-T", bad, "-f", "x", "-n", "1"], capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 1 and "Invalid tunable" in r.stderr, (bad, r.stderr)
