@@ -43,15 +43,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
-
-
 def serve_bench(args, log):
     """Served BC queries: a pool of graphs of uneven size (R-MAT scales 12-20,
     edge factor 8-32, and bridged-block chains of 256-vertex Erdos-Renyi
@@ -59,24 +50,12 @@ def serve_bench(args, log):
     Returns p50/p99 latency (arrival to completion) and aggregate GTEPS."""
     import threading
     import bcmi
-    rng = np.random.default_rng(args.seed + 7)
-    graphs = []
-    for i in range(args.serve_graphs):
-        if i % 4 == 3:
-            nb = int(rng.integers(64, args.serve_max_blocks + 1))
-            graphs.append(bcmi.Graph.block_chain(nb, 256, 8.0, seed=args.seed + 100 + i))
-        else:
-            sc = int(rng.integers(12, 21))
-            ef = int(rng.integers(8, 33))
-            graphs.append(bcmi.Graph.rmat(sc, ef, seed=args.seed + 100 + i))
+    import serve_pool
+    graphs, _, rng = serve_pool.make_pool(args.seed, args.serve_graphs, args.serve_max_blocks)
     srv = bcmi.Server(args.serve_budget_gb << 30, args.serve_slots)
     ids = [srv.add_graph(g) for g in graphs]
     nq = args.serve_queries
-    gsel = rng.integers(0, len(graphs), nq)
-    ks = rng.integers(16, 1025, nq)
-    gaps = rng.exponential(1.0 / args.serve_rate, nq)
-    qsrc = [graphs[gi].sample_sources(min(int(k), int((graphs[gi].degrees() > 0).sum())), seed=args.seed + 1000 + i)
-            for i, (gi, k) in enumerate(zip(gsel, ks))]
+    gsel, qsrc, gaps = serve_pool.make_queries(rng, graphs, args.seed, nq, args.serve_rate)
     # warm each graph's workspace once (first use captures the level loops)
     for gid, g in zip(ids, graphs):
         q = srv.submit(gid, g.sample_sources(1, seed=1))
@@ -216,10 +195,9 @@ def main():
         e.run(src[:args.warmup * width], mode=bcmi.MODE_BATCHED)
     e.reset()
     timed = src[args.warmup * width: need]
-    sync()
+    # bcmi_bc_run returns after its last kernel and copy have completed
     t0 = time.perf_counter()
     st = e.run(timed, mode=bcmi.MODE_BATCHED)
-    sync()
     t_batched = time.perf_counter() - t0
     k_ms, k_launch = e.kernel_time()
     ss = e.sweep_stats(len(timed))
@@ -259,10 +237,8 @@ def main():
     try:
         e.run(plain_src[:4], mode=bcmi.MODE_PLAIN)  # warm (capture)
         e.reset()
-        sync()
         t0 = time.perf_counter()
         stp = e.run(plain_src, mode=bcmi.MODE_PLAIN)
-        sync()
         t_plain = time.perf_counter() - t0
         gteps_plain = stp["te"] / t_plain / 1e9
         log(f"plain: {len(plain_src)} sources (the first of the timed list) in {t_plain:.3f}s "
@@ -321,6 +297,7 @@ def main():
         "plain_gteps": round(gteps_plain, 3) if gteps_plain else None,
         "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain else None,
         "model_bytes_gbs": round(st["bytes"] / t_batched / 1e9, 1),
+        "model_bytes_frac": round(st["bytes"] / t_batched / 1e9 / HBM_PEAK_GBS, 4),
         "levels_per_vertex_batch": round(st["sum_pairs"] / max(1, st["sum_union"]), 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -12,6 +12,7 @@
 // No counterpart in the reference (one program run per graph, bc2d.c:673).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -87,40 +88,61 @@ static int64_t active_slots(bcmi_server* s) {
   return a;
 }
 
-// Picks (or makes room for) a slot for graph gid.  Returns the slot, or
-// nullptr with *never set when the query can never fit the budget.
-static Slot* take_slot(bcmi_server* s, int32_t gid, bool* never, std::string* why) {
-  *never = false;
+// Frees the least recently used idle workspace; false when none is idle.
+static bool evict_one(bcmi_server* s) {
+  int victim = -1;
+  for (size_t i = 0; i < s->slots.size(); i++)
+    if (!s->slots[i]->busy && (victim < 0 || s->slots[i]->last_used < s->slots[victim]->last_used)) victim = (int)i;
+  if (victim < 0) return false;
+  workspace_destroy(s->slots[victim]->ws);
+  s->slot_bytes -= (int64_t)s->slots[victim]->bytes;
+  s->slots.erase(s->slots.begin() + victim);
+  return true;
+}
+
+enum Take { TAKEN, WAIT, FAIL };
+
+// Picks (or makes room for) a slot for graph gid.  WAIT: a running query must
+// free memory first.  FAIL: the query cannot run (*rc and *why say why): it
+// can never fit the budget, or its workspace cannot be allocated although no
+// other query holds device memory.  A new slot is charged the larger of the
+// graph's workspace estimate (which covers what the sweeps allocate later:
+// helper workspaces, workgroup-sweep state) and what was allocated.
+static Take take_slot(bcmi_server* s, int32_t gid, Slot** out, int* rc, std::string* why) {
+  *out = nullptr;
   for (auto& sl : s->slots)
-    if (!sl->busy && sl->gid == gid) return sl.get();
-  size_t need = s->graph_ws_bytes[gid];
-  if ((int64_t)need + s->graph_bytes > s->budget) {
-    *never = true;
+    if (!sl->busy && sl->gid == gid) { *out = sl.get(); return TAKEN; }
+  const int64_t need = (int64_t)s->graph_ws_bytes[gid];
+  if (need + s->graph_bytes > s->budget) {
+    *rc = BCMI_ERR_NOMEM;
     *why = "query needs more device memory than the HBM budget";
-    return nullptr;
+    return FAIL;
   }
   // evict idle workspaces of other graphs, least recently used first
-  while (s->graph_bytes + s->slot_bytes + (int64_t)need > s->budget) {
-    int victim = -1;
-    for (size_t i = 0; i < s->slots.size(); i++)
-      if (!s->slots[i]->busy && (victim < 0 || s->slots[i]->last_used < s->slots[victim]->last_used))
-        victim = (int)i;
-    if (victim < 0) return nullptr;  // wait for a busy slot to finish
-    workspace_destroy(s->slots[victim]->ws);
-    s->slot_bytes -= (int64_t)s->slots[victim]->bytes;
-    s->slots.erase(s->slots.begin() + victim);
+  while (s->graph_bytes + s->slot_bytes + need > s->budget)
+    if (!evict_one(s)) return WAIT;  // a busy slot must finish first
+  Workspace* ws = nullptr;
+  for (;;) {
+    int wrc = BCMI_OK;
+    ws = workspace_create(*s->graphs[gid], BCMI_MODE_BATCHED, 64, s->prm, &wrc);
+    if (ws) break;
+    // The device could not hold it: idle workspaces give their memory back
+    // first; then only a running query can free more.
+    if (evict_one(s)) continue;
+    if (active_slots(s) > 0) return WAIT;
+    *rc = wrc;
+    *why = std::string("workspace allocation failed: ") + bcmi_last_error();
+    return FAIL;
   }
-  int rc = BCMI_OK;
-  Workspace* ws = workspace_create(*s->graphs[gid], BCMI_MODE_BATCHED, 64, s->prm, &rc);
-  if (!ws) return nullptr;  // transient: retry when a slot frees up
   std::unique_ptr<Slot> sl(new Slot());
   sl->ws = ws;
   sl->gid = gid;
-  sl->bytes = workspace_size(ws);
+  sl->bytes = std::max<size_t>(workspace_size(ws), (size_t)need);
   s->slot_bytes += (int64_t)sl->bytes;
   s->peak_bytes = std::max<int64_t>(s->peak_bytes, s->slot_bytes + s->graph_bytes);
   s->slots.push_back(std::move(sl));
-  return s->slots.back().get();
+  *out = s->slots.back().get();
+  return TAKEN;
 }
 
 static void scheduler(bcmi_server* s) {
@@ -131,18 +153,19 @@ static void scheduler(bcmi_server* s) {
     if (!s->pending.empty() && active_slots(s) < s->max_slots) {
       Query* q = s->pending.front();
       std::string why;
-      bool never = false;
-      Slot* sl = take_slot(s, q->gid, &never, &why);
-      if (never) {
+      int rc = BCMI_OK;
+      Slot* sl = nullptr;
+      const Take t = take_slot(s, q->gid, &sl, &rc, &why);
+      if (t == FAIL) {
         s->pending.pop_front();
-        q->status = BCMI_ERR_NOMEM;
+        q->status = rc;
         q->err = why;
         q->start = q->finish = now_ms() - s->t0;
         q->done = true;
         s->completed++;
         s->cv_done.notify_all();
         progressed = true;
-      } else if (sl) {
+      } else if (t == TAKEN) {
         s->pending.pop_front();
         sl->busy = true;
         q->start = now_ms() - s->t0;
@@ -151,6 +174,8 @@ static void scheduler(bcmi_server* s) {
         s->cv_work.notify_one();
         progressed = true;
       }
+      // WAIT: a busy slot exists (take_slot checked), and its worker
+      // notifies cv_sched when it finishes
     }
     if (!progressed) s->cv_sched.wait(lk);
   }
@@ -189,6 +214,14 @@ static void worker(bcmi_server* s) {
     q->finish = now_ms() - s->t0;
     q->done = true;
     s->completed++;
+    // the sweeps may have grown the workspace (work lists, level tables,
+    // workgroup-sweep state): charge what it holds now
+    const size_t held = workspace_size(sl->ws);
+    if (held > sl->bytes) {
+      s->slot_bytes += (int64_t)(held - sl->bytes);
+      sl->bytes = held;
+      s->peak_bytes = std::max<int64_t>(s->peak_bytes, s->slot_bytes + s->graph_bytes);
+    }
     sl->busy = false;
     sl->last_used = now_ms();
     s->cv_done.notify_all();
@@ -222,6 +255,14 @@ int bcmi_server_destroy(bcmi_server* s) {
     // finish what was admitted; drop what is still queued
     s->cv_done.wait(lk, [&] { return s->assigned.empty() && active_slots(s) == 0; });
     s->stop = true;
+    // queries never admitted complete with an error (their waiters return)
+    for (Query* q : s->pending) {
+      q->status = BCMI_ERR_STATE;
+      q->err = "server destroyed before the query was admitted";
+      q->done = true;
+    }
+    s->pending.clear();
+    s->cv_done.notify_all();
   }
   s->cv_sched.notify_all();
   s->cv_work.notify_all();

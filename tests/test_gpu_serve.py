@@ -110,3 +110,44 @@ def test_budget_admission_and_rejection():
     rc, _, _ = s.wait(q, gs[0].n)
     assert rc == bcmi.ERR_NOMEM
     s.close()
+
+
+def test_allocation_failure_with_nothing_running():
+    """The budget admits the query but the device cannot hold its workspace
+    (the test holds nearly all device memory itself).  With no other query
+    running, nothing will free memory, so the query must complete with
+    BCMI_ERR_NOMEM instead of waiting; once the memory is back, the same
+    query runs."""
+    import ctypes
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+
+    def free_bytes():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value
+
+    g = bcmi.Graph.rmat(18, 16, seed=11)
+    src = g.sample_sources(64, seed=2)
+    s = Server(1 << 40, 2)
+    gid = s.add(g)
+    held = []
+    try:
+        # leave ~64 MiB: the 268 MB of path-count and coefficient rows cannot fit
+        for size in (8 << 30, 1 << 30, 128 << 20, 16 << 20):
+            while free_bytes() - size >= (64 << 20):
+                p = ctypes.c_void_p()
+                if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(size)) != 0:
+                    hip.hipGetLastError()
+                    break
+                held.append(p)
+        print(f"holding {sum(1 for _ in held)} blocks, {free_bytes() >> 20} MiB free", flush=True)
+        rc, _, res = s.wait(s.submit(gid, src), g.n)
+        assert rc == bcmi.ERR_NOMEM, rc
+        assert "allocation" in bcmi.lib().bcmi_last_error().decode()
+    finally:
+        for p in held:
+            hip.hipFree(p)
+    rc, bc, _ = s.wait(s.submit(gid, src), g.n)
+    assert rc == 0, bcmi.lib().bcmi_last_error()
+    assert bcmi.rel_err(bc, O.brandes(g.off, g.adj, src, nthreads=4)) <= 1e-9
+    s.close()
