@@ -117,14 +117,18 @@ def tune(g, src_batched, src_plain, log=print):
     return best_overall[1]
 
 
+# engine parameters a table entry may set (the rest of an entry are notes)
+APPLIED = ("chunk", "block", "grid", "grid_back", "grid_plain", "alpha_batched", "beta_batched", "gamma_push",
+           "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width")
+
+
 def apply(engine, table, key):
     """Sets the tuned parameters of `key` (or table['default']) on an engine
     whose graph is not uploaded yet.  Returns the parameters used."""
     p = table.get(key) or table.get("default")
     if not p:
         return None
-    for k in ("chunk", "block", "grid", "grid_back", "grid_plain", "alpha_batched", "beta_batched", "gamma_push",
-              "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width"):
+    for k in APPLIED:
         if k in p:
             engine.set_param(k, int(p[k]))
     return p

@@ -29,9 +29,9 @@ EXPORTS = [
     "bcmi_gen_block_chain", "bcmi_sample_sources", "bcmi_graph_free",
     "bcmi_create", "bcmi_destroy", "bcmi_set_param", "bcmi_get_param", "bcmi_upload_graph",
     "bcmi_get_peel", "bcmi_get_labels", "bcmi_bc_reset", "bcmi_bc_run", "bcmi_get_bc",
-    "bcmi_get_sweep_stats", "bcmi_get_kernel_time", "bcmi_sssp", "bcmi_msbfs",
+    "bcmi_get_sweep_stats", "bcmi_get_sweep_times", "bcmi_get_kernel_time", "bcmi_sssp", "bcmi_msbfs",
     "bcmi_server_create", "bcmi_server_destroy", "bcmi_server_add_graph", "bcmi_server_submit",
-    "bcmi_server_wait", "bcmi_server_stats",
+    "bcmi_server_wait", "bcmi_server_stats", "bcmi_launch_plan",
 ]
 
 
@@ -56,6 +56,11 @@ class RunStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Launch(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_char * 24), ("grid", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("aux", ctypes.c_int64), ("timed", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class QueryResult(ctypes.Structure):
@@ -101,6 +106,7 @@ def lib():
         L.bcmi_bc_run.argtypes = [vp, vp, i64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(RunStats)]
         L.bcmi_get_bc.argtypes = [vp, vp]
         L.bcmi_get_sweep_stats.argtypes = [vp, vp, vp, vp, vp]
+        L.bcmi_get_sweep_times.argtypes = [vp, vp]
         L.bcmi_get_kernel_time.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
         L.bcmi_sssp.argtypes = [vp, i32, vp, vp, vp]
         L.bcmi_msbfs.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp]
@@ -110,6 +116,9 @@ def lib():
         L.bcmi_server_submit.argtypes = [vp, i32, vp, i64, ctypes.POINTER(i64)]
         L.bcmi_server_wait.argtypes = [vp, i64, vp, ctypes.POINTER(QueryResult)]
         L.bcmi_server_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.bcmi_launch_plan.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(i64), ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.POINTER(Launch), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int)]
         _LIB = L
     return _LIB
 
@@ -289,6 +298,12 @@ class Engine:
         _check(lib().bcmi_get_sweep_stats(self._c, _p(n_s), _p(m_s), _p(a_s), _p(lv)))
         return dict(n_s=n_s, m_s=m_s, a_s=a_s, levels=lv)
 
+    def sweep_times(self, k):
+        """Per-sweep time (ms) of the last run, in the order of sweep_stats."""
+        t = np.zeros(k, np.float64)
+        _check(lib().bcmi_get_sweep_times(self._c, _p(t)))
+        return t
+
     def kernel_time(self):
         ms, n = ctypes.c_double(), ctypes.c_int64()
         _check(lib().bcmi_get_kernel_time(self._c, ctypes.byref(ms), ctypes.byref(n)))
@@ -309,6 +324,22 @@ class Engine:
         dl = np.zeros((k, self.n), np.float64)
         _check(lib().bcmi_msbfs(self._c, _p(src), k, _p(d), _p(sg), _p(dl)))
         return d, sg, dl
+
+
+def launch_plan(params, mode, graph):
+    """Kernel launches of one captured level step for the given tunables
+    (dict of bcmi_set_param keys); graph 0 forward, 1 backward, 2/3 the same
+    on sparse levels.  Returns (list of dicts, steps per graph).  No device."""
+    keys = list(params)
+    karr = (ctypes.c_char_p * max(1, len(keys)))(*[k.encode() for k in keys])
+    varr = (ctypes.c_int64 * max(1, len(keys)))(*[int(params[k]) for k in keys])
+    out = (Launch * 64)()
+    cnt, steps = ctypes.c_int(), ctypes.c_int()
+    _check(lib().bcmi_launch_plan(karr, varr, len(keys), int(mode), int(graph), out, 64, ctypes.byref(cnt),
+                                  ctypes.byref(steps)))
+    assert cnt.value <= 64
+    return [dict(kernel=out[i].kernel.decode(), grid=out[i].grid, block=out[i].block, aux=out[i].aux,
+                 timed=bool(out[i].timed)) for i in range(cnt.value)], steps.value
 
 
 def rel_err(a, b):

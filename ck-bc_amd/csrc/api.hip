@@ -58,44 +58,9 @@ static int get_ws(bcmi_ctx* c, int mode, Workspace** out) {
   return BCMI_OK;
 }
 
-extern "C" {
-
-const char* bcmi_last_error(void) { return last_error(); }
-int bcmi_version(void) { return 1; }
-
-int bcmi_create(int device, bcmi_ctx** out) {
-  if (!out) return set_error(BCMI_ERR_ARG, "null output");
-  int ndev = 0;
-  hipError_t e = hipGetDeviceCount(&ndev);
-  if (e != hipSuccess || ndev == 0) return set_error(BCMI_ERR_HIP, "no HIP device available");
-  if (device < 0 || device >= ndev) return set_error(BCMI_ERR_ARG, "device index out of range");
-  bcmi_ctx* c = new bcmi_ctx();
-  c->device = device;
-  int rc = use_device(c);
-  if (rc) { delete c; return rc; }
-  e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete c; return set_error(BCMI_ERR_HIP, "stream creation failed"); }
-  *out = c;
-  return BCMI_OK;
-}
-
-int bcmi_destroy(bcmi_ctx* c) {
-  CHECK_CTX(c);
-  use_device(c);
-  drop_workspaces(c);
-  c->g.release();
-  if (c->bc) (void)hipFree(c->bc);
-  if (c->st) (void)hipStreamDestroy(c->st);
-  delete c;
-  return BCMI_OK;
-}
-
-int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
-  CHECK_CTX(c);
-  if (!key) return set_error(BCMI_ERR_ARG, "null key");
-  std::lock_guard<std::mutex> lk(c->mu);
-  Params& p = c->prm;
-  std::string k(key);
+// Validates and sets one tunable (bcmi_set_param's keys).  have_graph: a
+// graph is uploaded (parameters fixed at upload may no longer change).
+static int param_set(Params& p, const std::string& k, int64_t v, bool have_graph) {
   if (k == "block") {
     if (v < 64 || v > 256 || v % 64) return set_error(BCMI_ERR_ARG, "block must be 64, 128 or 256");
     p.block = (int)v;
@@ -110,7 +75,7 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
     p.grid_plain = (int)v;
   } else if (k == "chunk") {
     if (v < 64 || v > (1 << 20) || v % 64) return set_error(BCMI_ERR_ARG, "chunk must be a multiple of 64");
-    if (c->have_graph) return set_error(BCMI_ERR_STATE, "chunk must be set before uploading a graph");
+    if (have_graph) return set_error(BCMI_ERR_STATE, "chunk must be set before uploading a graph");
     p.chunk = (int)v;
   } else if (k == "alpha_plain") {
     if (v < 0) return set_error(BCMI_ERR_ARG, "alpha must be >= 0");
@@ -171,7 +136,7 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
     if (v < 0 || v > 5) return set_error(BCMI_ERR_ARG, "batching must be 0..5");
     p.batching = (int)v;
   } else if (k == "reorder") {
-    if (c->have_graph) return set_error(BCMI_ERR_STATE, "reorder must be set before uploading a graph");
+    if (have_graph) return set_error(BCMI_ERR_STATE, "reorder must be set before uploading a graph");
     p.reorder = v ? 1 : 0;
   } else if (k == "trace") {
     p.trace = v ? 1 : 0;
@@ -180,6 +145,72 @@ int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
     p.ktime = (int)v;
   } else {
     return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
+  }
+  return BCMI_OK;
+}
+
+extern "C" {
+
+const char* bcmi_last_error(void) { return last_error(); }
+int bcmi_version(void) { return 1; }
+
+int bcmi_create(int device, bcmi_ctx** out) {
+  if (!out) return set_error(BCMI_ERR_ARG, "null output");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return set_error(BCMI_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return set_error(BCMI_ERR_ARG, "device index out of range");
+  bcmi_ctx* c = new bcmi_ctx();
+  c->device = device;
+  int rc = use_device(c);
+  if (rc) { delete c; return rc; }
+  e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return set_error(BCMI_ERR_HIP, "stream creation failed"); }
+  *out = c;
+  return BCMI_OK;
+}
+
+int bcmi_destroy(bcmi_ctx* c) {
+  CHECK_CTX(c);
+  use_device(c);
+  drop_workspaces(c);
+  c->g.release();
+  if (c->bc) (void)hipFree(c->bc);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return BCMI_OK;
+}
+
+int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
+  CHECK_CTX(c);
+  if (!key) return set_error(BCMI_ERR_ARG, "null key");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return param_set(c->prm, key, v, c->have_graph);
+}
+
+int bcmi_launch_plan(const char* const* keys, const int64_t* values, int nparams, int mode, int graph,
+                     bcmi_launch* out, int cap, int* count, int* steps) {
+  if (nparams < 0 || (nparams > 0 && (!keys || !values)) || !count || cap < 0 || (cap > 0 && !out))
+    return set_error(BCMI_ERR_ARG, "bad argument");
+  if (mode != BCMI_MODE_PLAIN && mode != BCMI_MODE_BATCHED) return set_error(BCMI_ERR_ARG, "bad mode");
+  if (graph < 0 || graph > 3) return set_error(BCMI_ERR_ARG, "graph must be 0..3");
+  Params p;
+  for (int i = 0; i < nparams; i++) {
+    if (!keys[i]) return set_error(BCMI_ERR_ARG, "null key");
+    int rc = param_set(p, keys[i], values[i], false);
+    if (rc) return rc;
+  }
+  const std::vector<StepNode> plan = step_plan(p, mode == BCMI_MODE_PLAIN, graph);
+  *count = (int)plan.size();
+  if (steps) *steps = p.steps;
+  for (int i = 0; i < cap && i < (int)plan.size(); i++) {
+    bcmi_launch& L = out[i];
+    memset(&L, 0, sizeof L);
+    strncpy(L.kernel, step_kernel_name(plan[i].kernel), sizeof L.kernel - 1);
+    L.grid = plan[i].grid;
+    L.block = plan[i].block;
+    L.aux = plan[i].aux;
+    L.timed = plan[i].timed ? 1 : 0;
   }
   return BCMI_OK;
 }
@@ -347,6 +378,14 @@ int bcmi_get_sweep_stats(bcmi_ctx* c, int64_t* n_s, int64_t* m_s, int64_t* a_s, 
   if (m_s) memcpy(m_s, R.m_s.data(), k * sizeof(int64_t));
   if (a_s) memcpy(a_s, R.a_s.data(), k * sizeof(int64_t));
   if (levels) memcpy(levels, R.levels.data(), R.levels.size() * sizeof(int32_t));
+  return BCMI_OK;
+}
+
+int bcmi_get_sweep_times(bcmi_ctx* c, double* ms) {
+  CHECK_CTX(c);
+  if (!ms) return set_error(BCMI_ERR_ARG, "null output");
+  const RunResult& R = c->last;
+  if (!R.t_s.empty()) memcpy(ms, R.t_s.data(), R.t_s.size() * sizeof(double));
   return BCMI_OK;
 }
 

@@ -16,6 +16,7 @@
 #include <getopt.h>
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cinttypes>
 #include <cstdarg>
@@ -311,23 +312,51 @@ int main(int argc, char** argv) {
       fclose(dbg);
     }
   }
-  int64_t nrounds = st.sources_run;
-  double mean = nrounds ? all_time / nrounds : 0.0;
-  // The reference's MTEPS: undirected edges of the reached component per
-  // source, divided by 2^20 (bc2d.c:565-579); here from the run totals.
-  double mteps = all_time > 0 ? (st.sum_m / 2.0) / all_time / 1048576.0 : 0.0;
+  // Per-source statistics as the reference keeps them (c:1037-1048,
+  // c:565-579): Max/Min over the per-source times (a batched sweep's time
+  // divided by its sources, each source getting that share), Mean = total /
+  // rounds, M-TEPS = mean over sources of (degree sum of the reached
+  // vertices / 2) / time / 2^20.  Rounds are the listed sources that were not
+  // skipped (c:1026-1037); under -H 1 a core source stands for its tree and
+  // the engine sweeps each core root once, so the statistics average over
+  // those sweeps.
+  const int64_t nrounds = run.empty() ? 0 : (int64_t)sources.size() - skipped;
+  const int64_t nsw = st.sources_run;
+  std::vector<double> ts(nsw > 0 ? nsw : 1), tsum(nsw > 0 ? nsw : 1);
+  std::vector<int64_t> ms(nsw > 0 ? nsw : 1);
+  double tmax = 0.0, tmin = 0.0, mteps_sum = 0.0;
+  if (nsw > 0) {
+    CK(bcmi_get_sweep_times(ctx, ts.data()));
+    CK(bcmi_get_sweep_stats(ctx, nullptr, ms.data(), nullptr, nullptr));
+    tmin = 1e300;
+    for (int64_t i = 0; i < nsw; i++) {
+      const double t = ts[i] / 1e3;
+      tmax = std::max(tmax, t);
+      if (t > 0) tmin = std::min(tmin, t);
+      if (t > 0) mteps_sum += (double)(ms[i] / 2) / t / 1048576.0;
+    }
+    if (tmin == 1e300) tmin = 0.0;
+  }
+  const double mean = nrounds ? all_time / nrounds : 0.0;
+  const double mteps = nsw ? mteps_sum / nsw : 0.0;
   printf("\n------------- RESULTS ---------------\n");
   printf("ClusterId\tSkip\tRounds\tExecTime\tRoundsTime\tMax\tMin\tMean\t1-dReduTime\n");
   printf("%d\t%" PRId64 "\t%" PRId64 "\t%lf\t%lf\t%lf\t%lf\t%lf\t%lf\n", 0, skipped, nrounds, all_time + bcred_time,
-         all_time, mean, mean, mean, redu_time);
+         all_time, tmax, tmin, mean, redu_time);
   printf("task 0 BC skipped: %" PRId64 " \n", skipped);
   printf("task 0 BC rounds: %" PRId64 "  \n", nrounds);
   printf("task 0 BC execution total time: %lf sec\n", all_time + bcred_time);
   printf("task 0 BC rounds total time: %lf sec\n", all_time);
-  printf("task 0 BC Max time: %lf sec\n", mean);
-  printf("task 0 BC Min time: %lf sec\n", mean);
+  printf("task 0 BC Max time: %lf sec\n", tmax);
+  printf("task 0 BC Min time: %lf sec\n", tmin);
   printf("task 0 BC Mean time: %lf sec\n", mean);
   printf("task 0 BC M-TEPS: %lf sec\n", mteps);
+  if (nbfs < N) {
+    // the whole graph's run time estimated from the sampled sources (c:1092-1097)
+    const uint64_t vskipped = (N - nbfs) * (uint64_t)skipped / nbfs;
+    printf("task 0 BC simulated time: %lf sec (virtual skipped %u)\n", mean * (double)(N - skipped - vskipped),
+           (unsigned)vskipped);
+  }
   printf("task 0 BC GTEPS (forward+backward arcs): %lf\n", all_time > 0 ? st.te / all_time / 1e9 : 0.0);
   if (heuristic == 1) printf("task 0 1-Degree reduction: %lf sec\n", redu_time);
   printf("\n");

@@ -109,6 +109,27 @@ struct DevGraph {
 
 struct Workspace;
 
+// Kernels of the captured level loop (one level step per graph node group).
+enum StepKernel {
+  SK_PLAIN_DECIDE, SK_PLAIN_TD, SK_PLAIN_BU, SK_BDECIDE, SK_PLAIN_BACK, SK_PLAIN_BFIN,
+  SK_MS_DECIDE, SK_MS_TD, SK_MS_BU, SK_MS_BU_PK, SK_MS_SIGMA, SK_MS_PUSH,
+  SK_MS_BDECIDE, SK_MS_BACK, SK_MS_BPUSH, SK_MS_BFIN, SK_KT_ACCUM, SK_KT_FOLD, SK_COUNT
+};
+struct StepNode {
+  int kernel;    // StepKernel
+  int grid;      // workgroups
+  int block;     // work-items per workgroup
+  int64_t aux;   // decide kernels: bottom-up allowed; k_kt_accum: waves of the timed launch
+  bool timed;    // the kernel stamps the device timer (Params::ktime)
+};
+// The kernel nodes of one captured level step; graph 0 forward, 1 backward,
+// 2/3 the same on sparse levels.  Host-only: needs no device.
+std::vector<StepNode> step_plan(const Params& prm, bool plain, int graph);
+const char* step_kernel_name(int kernel);
+// Per-wave stamp slots of the device timer: a timed launch has at most this
+// many waves (grid <= 65536 blocks of <= 256 work-items).
+constexpr int64_t kTimerMaxWaves = 65536 * 4;
+
 // Uploads and prepares a graph (peeling, components, work lists).
 int devgraph_build(DevGraph& g, int64_t n, const int64_t* off, const int32_t* adj, int directed,
                    int heuristic, const Params& prm, hipStream_t st);
@@ -129,6 +150,9 @@ struct RunResult {
   int64_t klaunches = 0;    // its launches (including ones that found no work)
   std::vector<int64_t> n_s, m_s, a_s;
   std::vector<int32_t> levels;
+  // per sweep (one entry per source, in the order of n_s): ms of a plain
+  // sweep, or of a batched / workgroup launch divided by its sources
+  std::vector<double> t_s;
 };
 
 Workspace* workspace_create(const DevGraph& g, int mode, int width, const Params& prm, int* rc);

@@ -151,7 +151,7 @@ struct KT {
   u64* wend;
   u64 plo[KT_PARTS], phi[KT_PARTS];  // partial first entry / last exit per accumulating block
 };
-#define KT_MAX_WAVES (65536 * 4)
+#define KT_MAX_WAVES kTimerMaxWaves
 static_assert(KT_PARTS == 64, "k_kt_fold folds the partials in one wave");
 __device__ __forceinline__ int64_t kt_wave() {
   return ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -2967,78 +2967,106 @@ static double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Captures `steps` forward levels and `steps` backward levels into two
-// graphs (kernel nodes only).
+const char* step_kernel_name(int k) {
+  static const char* names[] = {"k_plain_decide", "k_plain_td", "k_plain_bu", "k_bdecide", "k_plain_back",
+                                "k_plain_bfin", "k_ms_decide", "k_ms_td", "k_ms_bu", "k_ms_bu_pk", "k_ms_sigma",
+                                "k_ms_push", "k_ms_bdecide", "k_ms_back", "k_ms_bpush", "k_ms_bfin", "k_kt_accum",
+                                "k_kt_fold"};
+  static_assert(sizeof(names) / sizeof(names[0]) == SK_COUNT, "one name per step kernel");
+  return k >= 0 && k < SK_COUNT ? names[k] : "?";
+}
+
+// The kernel nodes of one level step of a captured graph (the graph holds
+// Params::steps copies).  graph: 0 forward, 1 backward, 2 and 3 the same on
+// sparse levels (small grid, top-down only).  The kernel selected by
+// Params::ktime is followed by the two timer nodes that fold its per-wave
+// stamps (k_kt_accum over grid x block / 64 waves, then k_kt_fold).
+std::vector<StepNode> step_plan(const Params& prm, bool plain, int graph) {
+  const bool fwd = (graph & 1) == 0;
+  const bool sparse = graph >= 2;
+  const int B = prm.block;
+  const int Gk = sparse ? prm.sgrid : plain ? prm.grid_plain : fwd || prm.grid_back <= 0 ? prm.grid : prm.grid_back;
+  std::vector<StepNode> v;
+  auto add = [&](int k, int grid, int block, int64_t aux, int timer_id) {
+    const bool timed = timer_id != 0 && prm.ktime == timer_id;
+    v.push_back({k, grid, block, aux, timed});
+    if (timed) {
+      v.push_back({SK_KT_ACCUM, KT_PARTS, 1024, (int64_t)grid * block / 64, false});
+      v.push_back({SK_KT_FOLD, 1, KT_PARTS, 0, false});
+    }
+  };
+  if (plain && fwd) {
+    add(SK_PLAIN_DECIDE, 1, 1, sparse ? 0 : 1, 0);
+    add(SK_PLAIN_TD, Gk, B, 0, 5);
+    if (!sparse) add(SK_PLAIN_BU, Gk, B, 0, 6);
+  } else if (plain) {
+    add(SK_BDECIDE, 1, 1, 0, 0);
+    add(SK_PLAIN_BACK, Gk, B, 0, 7);
+    add(SK_PLAIN_BFIN, Gk, B, 0, 0);
+  } else if (fwd) {
+    add(SK_MS_DECIDE, 1, 1, sparse ? 0 : 1, 0);
+    add(SK_MS_TD, Gk, B, 0, 3);
+    if (!sparse) add(SK_MS_BU, Gk, B, 0, 4);
+    if (!sparse && prm.pack_bu) add(SK_MS_BU_PK, Gk, B, 0, 4);
+    add(SK_MS_SIGMA, Gk, B, 0, 1);
+    add(SK_MS_PUSH, Gk, B, 0, 0);
+  } else {
+    add(SK_MS_BDECIDE, 1, 1, 0, 0);
+    add(SK_MS_BACK, Gk, B, 0, 2);
+    // with pushed levels the backward work is shared by three kernels, so
+    // the backward timer covers all of them
+    if (prm.gamma_back > 0) {
+      add(SK_MS_BPUSH, Gk, B, 0, 2);
+      add(SK_MS_BFIN, Gk, B, 0, 2);
+    } else {
+      add(SK_MS_BFIN, Gk, B, 0, 0);
+    }
+  }
+  return v;
+}
+
+// Captures `steps` forward levels and `steps` backward levels into graphs
+// (kernel nodes only), each step launched as step_plan lists it.
 static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
   ALLOC_LOCK;
   free_graphs(ws);
-  GA ga = make_ga(g, prm);
+  const GA ga = make_ga(g, prm);
+  const PA pa = make_pa(ws, prm);
+  const BA ba = make_ba(ws, prm);
   hipStream_t st = ws->st;
-  int B = prm.block, G = prm.grid;
-  // kernel timing: which kernel gets the device timer
-  const int kt = prm.ktime;
   const bool plain = ws->mode == BCMI_MODE_PLAIN;
-  KT* T = ws->kt;
-  auto tk = [&](int id) -> KT* { return kt == id ? T : nullptr; };
-  // after a timed kernel of Gk blocks: fold its wave stamps into the total
-  auto ktacc = [&](int id, int Gk) {
-    if (kt != id) return;
-    hipLaunchKernelGGL(k_kt_accum, KT_PARTS, 1024, 0, st, T, (int64_t)Gk * B / 64);
-    hipLaunchKernelGGL(k_kt_fold, 1, KT_PARTS, 0, st, T);
-  };
+  KT* const T = ws->kt;
   // Four graphs: {large grid, small grid} x {forward, backward}.  The small
   // one serves levels with few vertices (long chains): its launches are cheap
   // and it never considers bottom-up (which scans every vertex).
   for (int which = 0; which < 4; which++) {
-    const bool fwd = (which & 1) == 0;
-    const bool sparse = which >= 2;
-    const int Gk = sparse ? prm.sgrid : plain ? prm.grid_plain : fwd || prm.grid_back <= 0 ? G : prm.grid_back;
+    const std::vector<StepNode> plan = step_plan(prm, plain, which);
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < prm.steps; s++) {
-      if (plain) {
-        PA pa = make_pa(ws, prm);
-        if (fwd) {
-          hipLaunchKernelGGL(k_plain_decide, 1, 1, 0, st, ga, pa, sparse ? 0 : 1);
-          hipLaunchKernelGGL(k_plain_td, Gk, B, 0, st, ga, pa, tk(5));
-          ktacc(5, Gk);
-          if (!sparse) hipLaunchKernelGGL(k_plain_bu, Gk, B, 0, st, ga, pa, tk(6));
-          if (!sparse) ktacc(6, Gk);
-        } else {
-          hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, pa.ctl);
-          hipLaunchKernelGGL(k_plain_back, Gk, B, 0, st, ga, pa, tk(7));
-          ktacc(7, Gk);
-          hipLaunchKernelGGL(k_plain_bfin, Gk, B, 0, st, ga, pa);
-        }
-      } else {
-        BA ba = make_ba(ws, prm);
-        if (fwd) {
-          hipLaunchKernelGGL(k_ms_decide, 1, 1, 0, st, ga, ba, sparse ? 0 : 1);
-          hipLaunchKernelGGL(k_ms_td, Gk, B, 0, st, ga, ba, tk(3));
-          ktacc(3, Gk);
-          if (!sparse) hipLaunchKernelGGL(k_ms_bu, Gk, B, 0, st, ga, ba, tk(4));
-          if (!sparse) ktacc(4, Gk);
-          if (!sparse && prm.pack_bu) {
-            hipLaunchKernelGGL(k_ms_bu_pk, Gk, B, 0, st, ga, ba, tk(4));
-            ktacc(4, Gk);
-          }
-          hipLaunchKernelGGL(k_ms_sigma, Gk, B, 0, st, ga, ba, tk(1));
-          ktacc(1, Gk);
-          hipLaunchKernelGGL(k_ms_push, Gk, B, 0, st, ga, ba);
-        } else {
-          hipLaunchKernelGGL(k_ms_bdecide, 1, 1, 0, st, ga, ba);
-          hipLaunchKernelGGL(k_ms_back, Gk, B, 0, st, ga, ba, tk(2));
-          ktacc(2, Gk);
-          // With pushed levels the backward work is shared by three kernels,
-          // so the backward timer covers all of them.
-          if (prm.gamma_back > 0) {
-            hipLaunchKernelGGL(k_ms_bpush, Gk, B, 0, st, ga, ba, tk(2));
-            ktacc(2, Gk);
-            hipLaunchKernelGGL(k_ms_bfin, Gk, B, 0, st, ga, ba, tk(2));
-            ktacc(2, Gk);
-          } else {
-            hipLaunchKernelGGL(k_ms_bfin, Gk, B, 0, st, ga, ba, (KT*)nullptr);
-          }
+      for (const StepNode& nd : plan) {
+        KT* const kt = nd.timed ? T : nullptr;
+        const dim3 G(nd.grid), B(nd.block);
+        switch (nd.kernel) {
+          case SK_PLAIN_DECIDE: hipLaunchKernelGGL(k_plain_decide, G, B, 0, st, ga, pa, (int)nd.aux); break;
+          case SK_PLAIN_TD: hipLaunchKernelGGL(k_plain_td, G, B, 0, st, ga, pa, kt); break;
+          case SK_PLAIN_BU: hipLaunchKernelGGL(k_plain_bu, G, B, 0, st, ga, pa, kt); break;
+          case SK_BDECIDE: hipLaunchKernelGGL(k_bdecide, G, B, 0, st, pa.ctl); break;
+          case SK_PLAIN_BACK: hipLaunchKernelGGL(k_plain_back, G, B, 0, st, ga, pa, kt); break;
+          case SK_PLAIN_BFIN: hipLaunchKernelGGL(k_plain_bfin, G, B, 0, st, ga, pa); break;
+          case SK_MS_DECIDE: hipLaunchKernelGGL(k_ms_decide, G, B, 0, st, ga, ba, (int)nd.aux); break;
+          case SK_MS_TD: hipLaunchKernelGGL(k_ms_td, G, B, 0, st, ga, ba, kt); break;
+          case SK_MS_BU: hipLaunchKernelGGL(k_ms_bu, G, B, 0, st, ga, ba, kt); break;
+          case SK_MS_BU_PK: hipLaunchKernelGGL(k_ms_bu_pk, G, B, 0, st, ga, ba, kt); break;
+          case SK_MS_SIGMA: hipLaunchKernelGGL(k_ms_sigma, G, B, 0, st, ga, ba, kt); break;
+          case SK_MS_PUSH: hipLaunchKernelGGL(k_ms_push, G, B, 0, st, ga, ba); break;
+          case SK_MS_BDECIDE: hipLaunchKernelGGL(k_ms_bdecide, G, B, 0, st, ga, ba); break;
+          case SK_MS_BACK: hipLaunchKernelGGL(k_ms_back, G, B, 0, st, ga, ba, kt); break;
+          case SK_MS_BPUSH: hipLaunchKernelGGL(k_ms_bpush, G, B, 0, st, ga, ba, kt); break;
+          case SK_MS_BFIN: hipLaunchKernelGGL(k_ms_bfin, G, B, 0, st, ga, ba, kt); break;
+          case SK_KT_ACCUM: hipLaunchKernelGGL(k_kt_accum, G, B, 0, st, T, nd.aux); break;
+          case SK_KT_FOLD: hipLaunchKernelGGL(k_kt_fold, G, B, 0, st, T); break;
+          default: break;
         }
       }
     }
@@ -3264,6 +3292,7 @@ struct RunPart {
 
 // One plain sweep from source s (internal id) with BC multiplier mult.
 static int run_plain_one(const DevGraph& g, Workspace* w, const Params& prm, int32_t s, double mult, RunPart& P) {
+  const double t0 = now_ms();
   hipStream_t st = w->st;
   w->hsrc[0] = s;
   w->hmult[0] = mult;
@@ -3280,6 +3309,7 @@ static int run_plain_one(const DevGraph& g, Workspace* w, const Params& prm, int
   R.sweeps++; R.launches++;
   R.max_levels = std::max<int64_t>(R.max_levels, lv);
   R.levels.push_back(lv);
+  R.t_s.push_back(now_ms() - t0);
   if (c.sigma_ovf) P.ovf = 1;
   return BCMI_OK;
 }
@@ -3287,6 +3317,7 @@ static int run_plain_one(const DevGraph& g, Workspace* w, const Params& prm, int
 // One batched sweep over k <= 64 sources (internal ids) with multipliers.
 static int run_batch(const DevGraph& g, Workspace* w, const Params& prm, const int32_t* src, const double* mult,
                      int k, RunPart& P) {
+  const double t0 = now_ms();
   hipStream_t st = w->st;
   for (int q = 0; q < 64; q++) {
     w->hsrc[q] = q < k ? src[q] : 0;
@@ -3314,11 +3345,13 @@ static int run_batch(const DevGraph& g, Workspace* w, const Params& prm, const i
   if (scaled || w->hctl->scaled) R.scaled_batches++;
   HIP_TRY(hipMemcpyAsync(w->hstats, w->stats, 3 * 64 * sizeof(u64), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  const double per_src = (now_ms() - t0) / k;
   for (int q = 0; q < k; q++) {
     R.n_s.push_back((int64_t)w->hstats[q]);
     R.m_s.push_back((int64_t)w->hstats[64 + q]);
     R.a_s.push_back((int64_t)w->hstats[128 + q]);
     R.levels.push_back(lv);
+    R.t_s.push_back(per_src);
   }
   if (w->hctl->sigma_ovf) P.ovf = 1;
   R.uni_n.push_back(w->hctl->uni_verts);
@@ -3444,6 +3477,8 @@ static int wg_alloc_run(Workspace* ws, int64_t k) {
 // succeeded (an exact path count that wraps leaves the result untouched).
 static int run_wg(const DevGraph& g, Workspace* ws, const Params& prm, const std::vector<int32_t>& cs,
                   const std::vector<double>& cm, RunResult& R, int& ovf) {
+  const double t0 = now_ms();
+  const size_t first = R.n_s.size();
   hipStream_t st = ws->st;
   const int64_t lcap0 = std::max(wg_initial_lcap(g), ws->wg_lcap);
   if (!ws->wg_depth || ws->wg_lcap < lcap0 || ws->wg_req != wg_slot_count(g.n, prm, lcap0))
@@ -3513,6 +3548,9 @@ static int run_wg(const DevGraph& g, Workspace* ws, const Params& prm, const std
     mult.swap(rm);
   }
   hipLaunchKernelGGL(k_add_take_f64, grid_for(g.n), 256, 0, st, ws->bc, ws->wg_bc, g.n);
+  // the sources ran together: each gets an equal share of the run
+  const size_t done = R.n_s.size() - first;
+  if (done) R.t_s.insert(R.t_s.end(), done, (now_ms() - t0) / (double)done);
   return BCMI_OK;
 }
 
@@ -3629,6 +3667,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
     R.m_s.insert(R.m_s.end(), Q.m_s.begin(), Q.m_s.end());
     R.a_s.insert(R.a_s.end(), Q.a_s.begin(), Q.a_s.end());
     R.levels.insert(R.levels.end(), Q.levels.begin(), Q.levels.end());
+    R.t_s.insert(R.t_s.end(), Q.t_s.begin(), Q.t_s.end());
     R.uni_n.insert(R.uni_n.end(), Q.uni_n.begin(), Q.uni_n.end());
     R.uni_m.insert(R.uni_m.end(), Q.uni_m.begin(), Q.uni_m.end());
     R.pairs += Q.pairs;

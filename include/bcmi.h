@@ -188,6 +188,13 @@ int bcmi_get_bc(bcmi_ctx *ctx, double *bc);
  * degrees, DAG arcs, levels (arrays of length >= st->sources_run). */
 int bcmi_get_sweep_stats(bcmi_ctx *ctx, int64_t *n_s, int64_t *m_s, int64_t *a_s, int32_t *levels);
 
+/* Time per sweep of the last run, in ms, one entry per sweep in the order of
+ * bcmi_get_sweep_stats (length st->sources_run): a plain sweep's own time, or
+ * a batched sweep's (or a workgroup-per-source launch's) time divided by its
+ * sources.  Replaces the per-source bc_time the reference accumulates into
+ * its total, max and min (c:1037-1048, bc_func_mono c:504-580). */
+int bcmi_get_sweep_times(bcmi_ctx *ctx, double *ms);
+
 /* Duration (ms; per launch, first wave entry to last wave exit on the
  * device's constant-rate clock, summed on the device inside the captured
  * level loop)
@@ -207,6 +214,26 @@ int bcmi_sssp(bcmi_ctx *ctx, int32_t s, int32_t *depth, uint64_t *sigma, double 
  * out [source][vertex]. */
 int bcmi_msbfs(bcmi_ctx *ctx, const int32_t *sources, int k, int32_t *depth, uint64_t *sigma,
                double *delta);
+
+/* Diagnostic: the kernel launches of one level step of the captured level
+ * loop (a captured graph holds "steps" such steps), for the tunables given as
+ * nparams key/value pairs (bcmi_set_param keys) applied to the defaults.  No
+ * reference counterpart: the reference sizes and launches every kernel from
+ * the host, level by level (cu:861-907).  mode: BCMI_MODE_PLAIN or
+ * BCMI_MODE_BATCHED; graph: 0 forward, 1 backward, 2 forward on sparse
+ * levels, 3 backward on sparse levels.  Writes at most cap entries, sets
+ * *count to the step's node count and *steps to the steps per graph.  Needs
+ * no device. */
+typedef struct bcmi_launch {
+    char kernel[24];   /* kernel name */
+    int32_t grid;      /* workgroups */
+    int32_t block;     /* work-items per workgroup */
+    int64_t aux;       /* decide kernels: bottom-up allowed; k_kt_accum: waves of the timed launch */
+    int32_t timed;     /* the launch stamps the device kernel timer */
+    int32_t reserved;
+} bcmi_launch;
+int bcmi_launch_plan(const char *const *keys, const int64_t *values, int nparams, int mode, int graph,
+                     bcmi_launch *out, int cap, int *count, int *steps);
 
 /* ---- serving ------------------------------------------------------------ */
 typedef struct bcmi_server bcmi_server;

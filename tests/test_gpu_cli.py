@@ -136,3 +136,57 @@ def test_ck_run_generated_dataset_entry(tmp_path):
     ref = O.brandes(g.off, g.adj, np.arange(g.n)) / 2.0
     nz = ref != 0
     assert np.all(got[~nz] == 0) and np.max(np.abs(got[nz] - ref[nz]) / ref[nz]) < 1e-9
+
+
+def parse_results(stdout):
+    """The RESULTS block of the reference's stdout (bc2d.c:1074-1097)."""
+    lines = stdout.splitlines()
+    i = lines.index("------------- RESULTS ---------------")
+    assert lines[i + 1] == "ClusterId\tSkip\tRounds\tExecTime\tRoundsTime\tMax\tMin\tMean\t1-dReduTime"
+    row = lines[i + 2].split("\t")
+    out = dict(skip=int(row[1]), rounds=int(row[2]), exec=float(row[3]), rounds_time=float(row[4]),
+               max=float(row[5]), min=float(row[6]), mean=float(row[7]))
+    for l in lines[i + 3:]:
+        if l.startswith("task 0 BC M-TEPS: "):
+            out["mteps"] = float(l.split(": ")[1].split()[0])
+        if l.startswith("task 0 BC simulated time: "):
+            rest = l.split(": ", 1)[1]
+            out["simulated"] = float(rest.split()[0])
+            out["vskipped"] = int(rest.split("virtual skipped ")[1].rstrip(")"))
+    return out
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_results_block_statistics(tmp_path, mode):
+    """Per-source statistics as the reference computes them: Max and Min
+    over the sources' own times (c:1047-1048; a batched sweep's time shared
+    by its sources), Mean = rounds time / rounds, M-TEPS the mean of the
+    per-source rates (c:569-579, c:1091), and, with -N below the vertex
+    count, the simulated whole-graph time avg * (N - skipped - vskipped) with
+    vskipped = (N - nbfs) * skipped / nbfs (c:1092-1097)."""
+    # a graph with isolated vertices among the first sources: 0..9 isolated,
+    # a ring over 10..199
+    fn = tmp_path / "g.txt"
+    with open(fn, "w") as fp:
+        for v in range(10, 200):
+            fp.write(f"{v} {10 + (v - 9) % 190}\n")
+    r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "200", "-N", "100", "-B", mode],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    s = parse_results(r.stdout)
+    assert s["skip"] == 10 and s["rounds"] == 90
+    # times print with 1 us resolution
+    assert 0 < s["min"] <= s["max"] <= s["rounds_time"] + 1e-6
+    assert abs(s["mean"] - s["rounds_time"] / 90) <= 2e-6
+    # each source reaches the 190-vertex ring (degree 2): (380 / 2) / t / 2^20 per
+    # source, so the mean rate lies between the rates at the max and min times
+    lo, hi = 190 / (s["max"] + 1e-6) / 2**20, 190 / max(s["min"] - 1e-6, 1e-9) / 2**20
+    assert lo * 0.999 <= s["mteps"] <= hi * 1.001
+    vsk = (200 - 100) * 10 // 100
+    assert s["vskipped"] == vsk
+    # the engine multiplies the unrounded mean; the printed one is off by <= 0.5 us
+    assert abs(s["simulated"] - s["mean"] * (200 - 10 - vsk)) <= (200 - 10 - vsk) * 0.5e-6 + 1e-6
+    # all sources: no simulated line
+    r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "200", "-N", "200", "-B", mode],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "simulated" not in parse_results(r.stdout)
