@@ -173,7 +173,9 @@ int read_edgelist(const char* path, uint64_t n, int directed, bcmi_graph** out, 
   std::vector<size_t> cut(nt + 1, sz);
   cut[0] = 0;
   for (int t = 1; t < nt; t++) {
-    size_t c = sz * t / nt;
+    // a cut starts a line: scan from byte 1 at least (fewer bytes than
+    // threads put early cuts at 0, and data[c - 1] must stay in the file)
+    size_t c = std::max<size_t>(sz * t / nt, std::min<size_t>(sz, 1));
     while (c < sz && data[c - 1] != '\n') c++;
     cut[t] = std::max(c, cut[t - 1]);
   }

@@ -2,8 +2,10 @@
 oracle's restatement of the reference pipeline, the synthetic generators, and
 the C ABI (library loads, exports every declared entry point, fails cleanly
 without a GPU).  No device calls."""
+import json
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -232,3 +234,29 @@ def test_cli_rejects_malformed_tunables():
         r = subprocess.run([exe, "-p", "1x1", "-T", bad, "-f", "x", "-n", "1"], capture_output=True, text=True,
                            timeout=60)
         assert r.returncode == 1 and "Invalid tunable" in r.stderr, (bad, r.stderr)
+
+
+def test_read_tiny_files_many_threads(tmp_path):
+    """Files with fewer bytes than parser threads (every early cut falls at
+    byte 0): the parser must stay inside the mapped file and still read
+    every edge, with or without a final newline."""
+    import subprocess
+    cases = {"a.txt": ("0 1\n", [[1], [0]]), "b.txt": ("0 1", [[1], [0]]), "c.txt": ("", [[], []]),
+             "d.txt": ("1 0\n0 1\n", [[1], [0]]), "e.txt": ("#\n", [[], []])}
+    for name, (text, _) in cases.items():
+        (tmp_path / name).write_text(text)
+    code = (
+        "import sys, json; sys.path[:0] = [%r, %r]\n"
+        "import bcmi\n"
+        "out = {}\n"
+        "for f in sys.argv[1:]:\n"
+        "    g = bcmi.Graph.read_edgelist(f, 2)\n"
+        "    out[f] = [g.adj[g.off[v]:g.off[v + 1]].tolist() for v in range(2)]\n"
+        "print(json.dumps(out))\n") % (ROOT, os.path.join(ROOT, "ck-bc_amd"))
+    env = dict(os.environ, OMP_NUM_THREADS="64")
+    r = subprocess.run([sys.executable, "-c", code] + [str(tmp_path / n) for n in cases], env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+    for name, (_, want) in cases.items():
+        assert got[str(tmp_path / name)] == want, name
