@@ -73,12 +73,9 @@ def test_pendant_trees_scale18():
     e.close()
 
 
-def test_rmat_scale22_all_sources_tuned():
-    """The batched workload as measured: all 16,384 sources in batches of 64
-    with the tuned launch shape.  BC is linear in the sources, so the whole
-    run equals the sum of two runs over its halves (batched differently);
-    a 512-source run matches the oracle's distance checksum; two sources
-    match the CPU oracle; plain and batched sweeps agree on 64 sources."""
+@pytest.fixture(scope="module")
+def rmat22():
+    """R-MAT scale 22 uploaded to an engine with the launch shape tuned for it."""
     import autotune
     t = time.time()
     g = bcmi.Graph.rmat(22, 16, seed=SEED)
@@ -87,6 +84,17 @@ def test_rmat_scale22_all_sources_tuned():
     e = bcmi.Engine()
     assert autotune.apply(e, autotune.load(), autotune.graph_key("rmat", scale=22, edgefactor=16))
     e.upload(g)
+    yield g, src, e
+    e.close()
+
+
+def test_rmat_scale22_all_sources_tuned(rmat22):
+    """The batched workload as measured: all 16,384 sources in batches of 64
+    with the tuned launch shape.  BC is linear in the sources, so the whole
+    run equals the sum of two runs over its halves (batched differently);
+    a 512-source run matches the oracle's distance checksum; two sources
+    match the CPU oracle; plain and batched sweeps agree on 64 sources."""
+    g, src, e = rmat22
     e.reset()
     st = e.run(src, mode=bcmi.MODE_BATCHED)
     assert st["batches"] == 256
@@ -117,7 +125,32 @@ def test_rmat_scale22_all_sources_tuned():
     e.run(sixty4, mode=bcmi.MODE_PLAIN)
     assert bcmi.rel_err(bb, e.bc()) <= 1e-9
     say("rmat22: oracle and plain-vs-batched done")
-    e.close()
+
+
+def test_rmat_scale22_depth_sigma_bit_exact(rmat22):
+    """Forward rule (cuda_kernels.cu:524-537) at full size: one batch of 64
+    sources swept with the tuned launch shape and read back per source;
+    depths and path counts of 8 of its lanes equal the oracle's bit for bit,
+    their dependencies within 1e-12.  Two plain sweeps likewise."""
+    g, src, e = rmat22
+    batch = src[:64]
+    d, sg, dl = e.msbfs(batch)
+    say("rmat22: batch read back")
+    for lane in range(0, 64, 9):
+        od, osg, odl, _ = O.sssp(g.off, g.adj, int(batch[lane]))
+        assert np.array_equal(d[lane], od), lane
+        assert np.array_equal(sg[lane], osg), lane
+        odl[batch[lane]] = 0.0
+        assert bcmi.rel_err(dl[lane], odl) <= 1e-12, lane
+    del d, sg, dl
+    say("rmat22: 8 lanes of the batch bit-exact")
+    for s in src[64:66]:
+        d, sg, dl = e.sssp(int(s))
+        od, osg, odl, _ = O.sssp(g.off, g.adj, int(s))
+        assert np.array_equal(d, od) and np.array_equal(sg, osg)
+        odl[s] = 0.0
+        assert bcmi.rel_err(dl, odl) <= 1e-12
+    say("rmat22: plain sweeps bit-exact")
 
 
 def test_rmat_scale25():
