@@ -28,7 +28,7 @@ EXPORTS = [
     "bcmi_graph_from_edges", "bcmi_gen_rmat", "bcmi_gen_pendant_trees",
     "bcmi_gen_block_chain", "bcmi_sample_sources", "bcmi_graph_free",
     "bcmi_create", "bcmi_destroy", "bcmi_set_param", "bcmi_get_param", "bcmi_upload_graph",
-    "bcmi_get_peel", "bcmi_get_labels", "bcmi_bc_reset", "bcmi_bc_run", "bcmi_get_bc",
+    "bcmi_get_peel", "bcmi_get_labels", "bcmi_bc_reset", "bcmi_bc_run", "bcmi_get_bc", "bcmi_bc_run_static",
     "bcmi_get_sweep_stats", "bcmi_get_sweep_times", "bcmi_get_kernel_time", "bcmi_sssp", "bcmi_msbfs",
     "bcmi_server_create", "bcmi_server_destroy", "bcmi_server_add_graph", "bcmi_server_submit",
     "bcmi_server_wait", "bcmi_server_stats", "bcmi_launch_plan",
@@ -105,6 +105,7 @@ def lib():
         L.bcmi_bc_reset.argtypes = [vp]
         L.bcmi_bc_run.argtypes = [vp, vp, i64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(RunStats)]
         L.bcmi_get_bc.argtypes = [vp, vp]
+        L.bcmi_bc_run_static.argtypes = [vp, vp, i64, ctypes.POINTER(RunStats)]
         L.bcmi_get_sweep_stats.argtypes = [vp, vp, vp, vp, vp]
         L.bcmi_get_sweep_times.argtypes = [vp, vp]
         L.bcmi_get_kernel_time.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
@@ -283,6 +284,14 @@ class Engine:
         st = RunStats()
         _check(lib().bcmi_bc_run(self._c, _p(src), src.shape[0], int(mode), SRC_EXPAND if expand else SRC_EXACT,
                                  ctypes.byref(st)))
+        return st.as_dict()
+
+    def run_static(self, sources):
+        """Plain sweeps over the sources with no device allocation (state in the
+        library's static per-GPU scratch); accumulates like run(mode=MODE_PLAIN)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        st = RunStats()
+        _check(lib().bcmi_bc_run_static(self._c, _p(src), src.shape[0], ctypes.byref(st)))
         return st.as_dict()
 
     def bc(self):

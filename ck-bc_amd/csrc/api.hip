@@ -164,6 +164,8 @@ int bcmi_create(int device, bcmi_ctx** out) {
   c->device = device;
   int rc = use_device(c);
   if (rc) { delete c; return rc; }
+  rc = static_scratch_init(device);  // once per device: the address of its static sweep state
+  if (rc) { delete c; return rc; }
   e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return set_error(BCMI_ERR_HIP, "stream creation failed"); }
   *out = c;
@@ -355,6 +357,35 @@ int bcmi_bc_run(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int mode, int
     st->scaled_batches = R.scaled_batches;
     st->wg_sources = R.wg_sources;
     for (int64_t u : R.uni_n) st->sum_union += u;
+  }
+  return rc;
+}
+
+int bcmi_bc_run_static(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, bcmi_run_stats* st) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  if (nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad source list");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = use_device(c);
+  if (rc) return rc;
+  StaticRun S;
+  rc = bc_run_static(c->g, c->bc, c->st, c->device, c->prm, sources, nsrc, &S);
+  // the per-sweep records describe bcmi_bc_run's last run: drop them (clear keeps
+  // the vectors' storage, so nothing is freed or allocated)
+  c->last.n_s.clear(); c->last.m_s.clear(); c->last.a_s.clear(); c->last.levels.clear(); c->last.t_s.clear();
+  c->last.ktime_ms = 0; c->last.klaunches = 0;
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    st->sources_run = S.sweeps;
+    st->batches = S.sweeps;
+    st->te = S.te;
+    st->bytes = S.bytes;
+    st->gpu_ms = S.wall_ms;
+    st->wall_ms = S.wall_ms;
+    st->max_levels = S.max_levels;
+    st->sum_n = S.sum_n;
+    st->sum_m = S.sum_m;
+    st->sum_a = S.sum_a;
   }
   return rc;
 }

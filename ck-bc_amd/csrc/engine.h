@@ -168,6 +168,21 @@ int workspace_bc_reset(Workspace* ws);
 int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* sources, int64_t nsrc,
            int mode, int src_flags, RunResult* out);
 
+// Allocation-free plain sweeps: per-sweep state in a static device array of
+// kStaticScratchBytes per GPU.  static_scratch_init (once per device, with
+// that device current) looks up the array's address; bc_run_static then runs
+// plain sweeps on stream st into bc without allocating.
+constexpr size_t kStaticScratchBytes = BCMI_STATIC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+struct StaticRun {
+  int64_t sweeps = 0, sum_n = 0, sum_m = 0, sum_a = 0, max_levels = 0;
+  double te = 0, bytes = 0, wall_ms = 0;
+};
+int static_scratch_init(int device);
+size_t static_scratch_need(const DevGraph& g);
+int bc_run_static(const DevGraph& g, double* bc, hipStream_t st, int device, const Params& prm,
+                  const int32_t* sources, int64_t nsrc, StaticRun* out);
+
 // Debug sweeps (no BC accumulation).
 int sssp_debug(const DevGraph& g, Workspace* ws, const Params& prm, int32_t s, int32_t* depth,
                uint64_t* sigma, double* delta);

@@ -3722,6 +3722,147 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   return BCMI_OK;
 }
 
+// ====================================== allocation-free plain sweeps
+// Plain sweeps whose per-sweep state lives in one static device array per
+// GPU (`scratch`) instead of a workspace, for callers that may not allocate
+// device memory (code under graph capture, tight serving loops, harnesses
+// that forbid hipMalloc).  The kernels are the plain sweep's, launched
+// directly: instantiating a captured level loop would allocate.  The array
+// keeps the last call's contents, so every call first clears the state it
+// reads; calls on one device share it and are serialised here.
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+static std::mutex g_scratch_mu[kMaxDevices];  // one call at a time per device
+static std::mutex g_scratch_init_mu;
+static void* g_scratch_ptr[kMaxDevices];      // this device's copy of `scratch` (hipGetSymbolAddress)
+
+int static_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return set_error(BCMI_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+  if (g_scratch_ptr[device]) return BCMI_OK;
+  int cur = -1;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != device) return set_error(BCMI_ERR_STATE, "static scratch set up with another device current");
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_error(BCMI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+  return BCMI_OK;
+}
+
+// Byte offsets of the plain sweep's arrays inside the scratch (256-aligned).
+struct ScratchLayout {
+  size_t ctl, src, mult, rec_lo, depth, sigma, delta, coef, rec_v, rec_j, end;
+  int64_t cap;
+};
+static ScratchLayout scratch_layout(int64_t n, int64_t m, int chunk) {
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  ScratchLayout L;
+  L.cap = n + m / chunk + 1024;  // a plain sweep appends each vertex once (record_capacity)
+  size_t o = 0;
+  L.ctl = o; o = up(o + sizeof(Ctl));
+  L.src = o; o = up(o + sizeof(int32_t) * 64);
+  L.mult = o; o = up(o + sizeof(double) * 64);
+  L.rec_lo = o; o = up(o + sizeof(int64_t) * (size_t)(n + 3));
+  L.depth = o; o = up(o + sizeof(int32_t) * (size_t)n);
+  L.sigma = o; o = up(o + sizeof(u64) * (size_t)n);
+  L.delta = o; o = up(o + sizeof(double) * (size_t)n);
+  L.coef = o; o = up(o + sizeof(double) * (size_t)n);
+  L.rec_v = o; o = up(o + sizeof(int32_t) * (size_t)L.cap);
+  L.rec_j = o; o = up(o + sizeof(int32_t) * (size_t)L.cap);
+  L.end = o;
+  return L;
+}
+
+size_t static_scratch_need(const DevGraph& g) { return scratch_layout(g.n, g.m, g.chunk).end; }
+
+int bc_run_static(const DevGraph& g, double* bc, hipStream_t st, int device, const Params& prm,
+                  const int32_t* sources, int64_t nsrc, StaticRun* out) {
+  *out = StaticRun();
+  const double t0 = now_ms();
+  if (device < 0 || device >= kMaxDevices || !g_scratch_ptr[device])
+    return set_error(BCMI_ERR_STATE, "static scratch not set up on this device");
+  if (g.heuristic) return set_error(BCMI_ERR_STATE, "peeled graphs run through bcmi_bc_run");
+  const ScratchLayout L = scratch_layout(g.n, g.m, g.chunk);
+  if (L.end > kStaticScratchBytes)
+    return set_error(BCMI_ERR_ARG, "graph needs " + std::to_string(L.end) + " bytes of sweep state; the static scratch holds " +
+                                       std::to_string(kStaticScratchBytes) + " (about 1.5M vertices at 16 arcs each)");
+  for (int64_t i = 0; i < nsrc; i++)
+    if (sources[i] < 0 || sources[i] >= g.n) return set_error(BCMI_ERR_RANGE, "source id out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu[device]);
+  char* base = (char*)g_scratch_ptr[device];
+  PA pa;
+  pa.ctl = (Ctl*)(base + L.ctl);
+  pa.depth = (int32_t*)(base + L.depth);
+  pa.sigma = (u64*)(base + L.sigma);
+  pa.delta = (double*)(base + L.delta);
+  pa.coef = (double*)(base + L.coef);
+  pa.done = nullptr;
+  pa.rec_v = (int32_t*)(base + L.rec_v);
+  pa.rec_j = (int32_t*)(base + L.rec_j);
+  pa.cap = L.cap;
+  pa.rec_lo = (int64_t*)(base + L.rec_lo);
+  pa.srcbuf = (const int32_t*)(base + L.src);
+  pa.multbuf = (const double*)(base + L.mult);
+  pa.bc = bc;
+  pa.alpha = prm.alpha_plain;
+  pa.beta = prm.beta_plain;
+  const GA ga = make_ga(g, prm);
+  const int64_t n = g.n;
+  const int G = grid_for(n);
+  // the previous call (maybe on another graph) left its state behind
+  hipLaunchKernelGGL(k_fill_i32, G, 256, 0, st, pa.depth, -1, n);
+  hipLaunchKernelGGL(k_fill_u64, G, 256, 0, st, pa.sigma, 0ull, n);
+  hipLaunchKernelGGL(k_fill_f64, G, 256, 0, st, pa.delta, 0.0, n);
+  hipLaunchKernelGGL(k_fill_f64, G, 256, 0, st, pa.coef, 0.0, n);
+  hipLaunchKernelGGL(k_fill_f64, 1, 64, 0, st, (double*)pa.multbuf, 1.0, (int64_t)1);
+  const int Gk = prm.grid_plain, B = prm.block, steps = prm.steps;
+  KT* const nokt = nullptr;
+  int ovf = 0;
+  for (int64_t i = 0; i < nsrc; i++) {
+    const int32_t s = g.to_int(sources[i]);
+    if (g.h_off[s + 1] == g.h_off[s]) continue;  // degree 0: skipped (bc2d.c:1026)
+    hipLaunchKernelGGL(k_fill_i32, 1, 64, 0, st, (int32_t*)pa.srcbuf, s, (int64_t)1);
+    hipLaunchKernelGGL(k_plain_init, 1, 1, 0, st, ga, pa);
+    Ctl hc;
+    for (;;) {  // forward levels, `steps` at a time, until the decide kernel closes the sweep
+      for (int t = 0; t < steps; t++) {
+        hipLaunchKernelGGL(k_plain_decide, 1, 1, 0, st, ga, pa, 1);
+        hipLaunchKernelGGL(k_plain_td, Gk, B, 0, st, ga, pa, nokt);
+        hipLaunchKernelGGL(k_plain_bu, Gk, B, 0, st, ga, pa, nokt);
+      }
+      HIP_TRY(hipMemcpyAsync(&hc, pa.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      if (hc.fdone) break;
+    }
+    if (hc.overflow) return set_error(BCMI_ERR_STATE, "plain sweep overflowed its work list");
+    while (!hc.bdone) {  // backward levels, deepest first
+      for (int t = 0; t < steps; t++) {
+        hipLaunchKernelGGL(k_bdecide, 1, 1, 0, st, pa.ctl);
+        hipLaunchKernelGGL(k_plain_back, Gk, B, 0, st, ga, pa, nokt);
+        hipLaunchKernelGGL(k_plain_bfin, Gk, B, 0, st, ga, pa);
+      }
+      HIP_TRY(hipMemcpyAsync(&hc, pa.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    hipLaunchKernelGGL(k_plain_cleanup, grid_for(std::min<int64_t>(hc.rec_count, L.cap)), 256, 0, st, pa);
+    if (hc.sigma_ovf) ovf = 1;
+    out->sweeps++;
+    out->sum_n += hc.tot_verts;
+    out->sum_m += hc.tot_arcs;
+    out->sum_a += hc.sum_a;
+    out->bytes += 16.0 * hc.tot_arcs + 24.0 * hc.sum_a + 64.0 * hc.tot_verts;
+    out->max_levels = std::max<int64_t>(out->max_levels, hc.last + 1);
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipGetLastError());
+  out->te = 2.0 * (double)out->sum_m;
+  out->wall_ms = now_ms() - t0;
+  if (ovf) return set_error(BCMI_ERR_OVERFLOW, "a shortest-path count exceeded 2^64-1");
+  return BCMI_OK;
+}
+
 int sssp_debug(const DevGraph& g, Workspace* ws, const Params& prm, int32_t s, int32_t* depth, uint64_t* sigma,
                double* delta) {
   if (ws->mode != BCMI_MODE_PLAIN) return set_error(BCMI_ERR_STATE, "plain workspace required");

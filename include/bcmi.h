@@ -181,6 +181,20 @@ int bcmi_bc_reset(bcmi_ctx *ctx);
  * Sources of degree 0 contribute nothing (c:1026-1035).  st may be NULL. */
 int bcmi_bc_run(bcmi_ctx *ctx, const int32_t *sources, int64_t nsrc, int mode, int src_flags,
                 bcmi_run_stats *st);
+/* Allocation-free variant of bcmi_bc_run in BCMI_MODE_PLAIN.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The sweep state lives in a static device array of
+ * BCMI_STATIC_SCRATCH_BYTES per GPU, set up by bcmi_create, instead of a
+ * workspace, so the call allocates nothing; graphs whose state does not fit
+ * (about 1.5M vertices at 16 arcs each) fail with BCMI_ERR_ARG, peeled graphs
+ * with BCMI_ERR_STATE.  It accumulates into the context's BC like
+ * bcmi_bc_run (same sources, same results) and fills st's totals (per-sweep
+ * statistics are not kept).  Every call on one GPU uses that GPU's array and
+ * leaves its contents behind; calls that overlap on one GPU would race on it,
+ * so the library runs them one at a time per GPU. */
+#define BCMI_STATIC_SCRATCH_BYTES (64ull << 20)
+int bcmi_bc_run_static(bcmi_ctx *ctx, const int32_t *sources, int64_t nsrc, bcmi_run_stats *st);
 /* Copies the BC accumulator to the host (replaces get_bc cu:1039-1041;
  * double instead of float, not halved). */
 int bcmi_get_bc(bcmi_ctx *ctx, double *bc);
