@@ -164,8 +164,6 @@ int bcmi_create(int device, bcmi_ctx** out) {
   c->device = device;
   int rc = use_device(c);
   if (rc) { delete c; return rc; }
-  rc = static_scratch_init(device);  // once per device: the address of its static sweep state
-  if (rc) { delete c; return rc; }
   e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return set_error(BCMI_ERR_HIP, "stream creation failed"); }
   *out = c;

@@ -169,9 +169,10 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
            int mode, int src_flags, RunResult* out);
 
 // Allocation-free plain sweeps: per-sweep state in a static device array of
-// kStaticScratchBytes per GPU.  static_scratch_init (once per device, with
-// that device current) looks up the array's address; bc_run_static then runs
-// plain sweeps on stream st into bc without allocating.
+// kStaticScratchBytes per GPU (static_scratch.hip).  static_scratch_init
+// (once per device, with that device current) looks up the array's address,
+// static_scratch_ptr returns it; bc_run_static runs plain sweeps on stream st
+// into bc without allocating (its first call on a device does the set-up).
 constexpr size_t kStaticScratchBytes = BCMI_STATIC_SCRATCH_BYTES;
 constexpr int kMaxDevices = 64;
 struct StaticRun {
@@ -179,6 +180,7 @@ struct StaticRun {
   double te = 0, bytes = 0, wall_ms = 0;
 };
 int static_scratch_init(int device);
+void* static_scratch_ptr(int device);
 size_t static_scratch_need(const DevGraph& g);
 int bc_run_static(const DevGraph& g, double* bc, hipStream_t st, int device, const Params& prm,
                   const int32_t* sources, int64_t nsrc, StaticRun* out);

@@ -3723,33 +3723,14 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
 }
 
 // ====================================== allocation-free plain sweeps
-// Plain sweeps whose per-sweep state lives in one static device array per
-// GPU (`scratch`) instead of a workspace, for callers that may not allocate
-// device memory (code under graph capture, tight serving loops, harnesses
-// that forbid hipMalloc).  The kernels are the plain sweep's, launched
-// directly: instantiating a captured level loop would allocate.  The array
-// keeps the last call's contents, so every call first clears the state it
-// reads; calls on one device share it and are serialised here.
-static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
-
+// Plain sweeps whose per-sweep state lives in the static device array of
+// static_scratch.hip (one per GPU) instead of a workspace, for callers that
+// may not allocate device memory (code under graph capture, tight serving
+// loops, harnesses that forbid hipMalloc).  The kernels are the plain
+// sweep's, launched directly: instantiating a captured level loop would
+// allocate.  The array keeps the last call's contents, so every call first
+// clears the state it reads; calls on one device are serialised here.
 static std::mutex g_scratch_mu[kMaxDevices];  // one call at a time per device
-static std::mutex g_scratch_init_mu;
-static void* g_scratch_ptr[kMaxDevices];      // this device's copy of `scratch` (hipGetSymbolAddress)
-
-int static_scratch_init(int device) {
-  if (device < 0 || device >= kMaxDevices) return set_error(BCMI_ERR_ARG, "device index out of range");
-  std::lock_guard<std::mutex> lk(g_scratch_init_mu);
-  if (g_scratch_ptr[device]) return BCMI_OK;
-  int cur = -1;
-  HIP_TRY(hipGetDevice(&cur));
-  if (cur != device) return set_error(BCMI_ERR_STATE, "static scratch set up with another device current");
-  void* ptr = nullptr;
-  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
-  if (e != hipSuccess || !ptr)
-    return set_error(BCMI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
-  g_scratch_ptr[device] = ptr;
-  return BCMI_OK;
-}
 
 // Byte offsets of the plain sweep's arrays inside the scratch (256-aligned).
 struct ScratchLayout {
@@ -3781,8 +3762,8 @@ int bc_run_static(const DevGraph& g, double* bc, hipStream_t st, int device, con
                   const int32_t* sources, int64_t nsrc, StaticRun* out) {
   *out = StaticRun();
   const double t0 = now_ms();
-  if (device < 0 || device >= kMaxDevices || !g_scratch_ptr[device])
-    return set_error(BCMI_ERR_STATE, "static scratch not set up on this device");
+  if (device < 0 || device >= kMaxDevices) return set_error(BCMI_ERR_ARG, "device index out of range");
+  TRY(static_scratch_init(device));  // the first call on this device looks the array up
   if (g.heuristic) return set_error(BCMI_ERR_STATE, "peeled graphs run through bcmi_bc_run");
   const ScratchLayout L = scratch_layout(g.n, g.m, g.chunk);
   if (L.end > kStaticScratchBytes)
@@ -3791,7 +3772,7 @@ int bc_run_static(const DevGraph& g, double* bc, hipStream_t st, int device, con
   for (int64_t i = 0; i < nsrc; i++)
     if (sources[i] < 0 || sources[i] >= g.n) return set_error(BCMI_ERR_RANGE, "source id out of range");
   std::lock_guard<std::mutex> lk(g_scratch_mu[device]);
-  char* base = (char*)g_scratch_ptr[device];
+  char* base = (char*)static_scratch_ptr(device);
   PA pa;
   pa.ctl = (Ctl*)(base + L.ctl);
   pa.depth = (int32_t*)(base + L.depth);

@@ -185,8 +185,9 @@ int bcmi_bc_run(bcmi_ctx *ctx, const int32_t *sources, int64_t nsrc, int mode, i
  * in the reference: it is for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The sweep state lives in a static device array of
- * BCMI_STATIC_SCRATCH_BYTES per GPU, set up by bcmi_create, instead of a
- * workspace, so the call allocates nothing; graphs whose state does not fit
+ * BCMI_STATIC_SCRATCH_BYTES per GPU instead of a workspace; the first call
+ * on a GPU looks the array up, which loads it (held until the process ends),
+ * and every call after that allocates nothing; graphs whose state does not fit
  * (about 1.5M vertices at 16 arcs each) fail with BCMI_ERR_ARG, peeled graphs
  * with BCMI_ERR_STATE.  It accumulates into the context's BC like
  * bcmi_bc_run (same sources, same results) and fills st's totals (per-sweep
