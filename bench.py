@@ -3,15 +3,17 @@
 A step is one batch of 64 sources (one forward + backward pass of the
 batched sweep over the graph).  The workload is R-MAT scale 22, edge factor
 16, a/b/c/d = .57/.19/.19/.05, ids permuted, 16,384 seeded sampled sources in
-batches of 64.  By default 2 warmup batches and 254 timed batches run, so the
-timed region covers the rest of the 16,384 sources (the engine groups the
-sources of one call into batches by similarity, so the whole set batches
-better than a part of it); the graph and state are already resident in HBM.
+batches of 64.  The batches are the ones the engine forms over all 16,384
+sources (it groups the sources of a run by similarity, so the whole set
+batches better than a part of it; bcmi_batch_order gives that grouping).  By
+default 2 warmup batches and the other 254 are timed; with --steps K below
+that, K batches spread evenly over the schedule are timed.  The graph and
+state are already resident in HBM.
 
 Printed JSON line (one): GTEPS of the batched path (traversed edges = 2 *
 sum over sources of the degree sum of the vertices each source reaches),
-the plain one-source-per-sweep GTEPS on the first `--plain-sources` of the
-same sources, the roofline of the dominant kernel (algorithmic bytes from the
+the plain one-source-per-sweep GTEPS on `--plain-sources` of the timed
+sources (spread over the timed batches), the roofline of the dominant kernel (algorithmic bytes from the
 frozen byte model / kernel duration from a device-side timer inside the
 captured level loop), the HBM traffic per launch of that kernel from the
 committed rocprofv3 counter runs, the CPU oracle timed on a bounded sample
@@ -189,12 +191,33 @@ def main():
     if need > len(src):
         src = g.sample_sources(need, seed=args.seed + 1)
     kt = {"sigma": 1, "back": 2, "td": 3, "bu": 4, "none": 0}[args.kernel]
+    nb = len(src) // width
+    if heur == bcmi.H_NONE:
+        # A step is one batch of the whole workload: the engine's grouping of all
+        # the sampled sources into batches of `width` (bcmi_batch_order).  With
+        # fewer steps than batches the timed ones are spread evenly over that
+        # schedule and the warmup batches are others, so every timed batch is
+        # one the full run would execute; "batching" 0 keeps them as formed.
+        order = e.batch_order(src)
+        batches = [order[i * width:(i + 1) * width] for i in range(nb)]
+        if args.warmup + args.steps >= nb:
+            warm_ix, timed_ix = range(args.warmup), range(args.warmup, args.warmup + args.steps)
+        else:
+            timed_ix = sorted({int((i + 0.5) * nb / args.steps) for i in range(args.steps)})
+            rest = [i for i in range(nb) if i not in set(timed_ix)]
+            warm_ix = rest[::max(1, len(rest) // max(1, args.warmup))][:args.warmup]
+        e.set_param("batching", 0)
+        warm = np.concatenate([batches[i] for i in warm_ix]) if args.warmup else np.zeros(0, np.int32)
+        timed = np.concatenate([batches[i] for i in timed_ix])
+        schedule = f"{len(timed_ix)} of the {nb} batches the engine forms over all {len(src)} sources"
+    else:
+        warm, timed = src[:args.warmup * width], src[args.warmup * width: need]
+        schedule = f"{len(timed)} of {len(src)} sources, batched among themselves"
     # warmup (also captures the level-loop graphs)
     e.set_param("ktime", kt)
     if args.warmup:
-        e.run(src[:args.warmup * width], mode=bcmi.MODE_BATCHED)
+        e.run(warm, mode=bcmi.MODE_BATCHED)
     e.reset()
-    timed = src[args.warmup * width: need]
     # bcmi_bc_run returns after its last kernel and copy have completed
     t0 = time.perf_counter()
     st = e.run(timed, mode=bcmi.MODE_BATCHED)
@@ -232,7 +255,8 @@ def main():
 
     # plain path on the same (first) sources
     e.set_param("ktime", 0)
-    plain_src = timed[:args.plain_sources]
+    # plain sweeps and the CPU sample on sources spread over the timed batches
+    plain_src = timed[np.linspace(0, len(timed) - 1, min(args.plain_sources, len(timed))).astype(np.int64)]
     gteps_plain = None
     try:
         e.run(plain_src[:4], mode=bcmi.MODE_PLAIN)  # warm (capture)
@@ -241,7 +265,7 @@ def main():
         stp = e.run(plain_src, mode=bcmi.MODE_PLAIN)
         t_plain = time.perf_counter() - t0
         gteps_plain = stp["te"] / t_plain / 1e9
-        log(f"plain: {len(plain_src)} sources (the first of the timed list) in {t_plain:.3f}s "
+        log(f"plain: {len(plain_src)} of the timed sources in {t_plain:.3f}s "
             f"-> {gteps_plain:.2f} GTEPS")
     except bcmi.BcmiError as ex:
         if ex.code != bcmi.ERR_OVERFLOW:
@@ -252,7 +276,7 @@ def main():
     if not args.no_cpu:
         from oracle import oracle as O
         nth = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cs = timed[:args.cpu_sources]
+        cs = timed[np.linspace(0, len(timed) - 1, min(args.cpu_sources, len(timed))).astype(np.int64)]
         # plain Brandes on the uploaded graph (no peeling); on chains the uint64
         # path counts wrap, which changes no work: the run is timed only
         O.brandes(g.off, g.adj, cs[:1], nthreads=1, allow_overflow=True)  # page in
@@ -289,7 +313,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u64 sigma / f64 delta",
         "data": "synthetic, seeded",
-        "config": {"workload": f"batched Brandes BC, {gdesc}, {len(timed)} sampled sources in batches of {width}",
+        "config": {"workload": f"batched Brandes BC, {gdesc}, sampled sources in batches of {width}: {schedule}",
                    "n": g.n, "arcs": g.m, "batch": width,
                    "tuned": {k: v for k, v in (tuned or {}).items() if k in
                              ("chunk", "block", "grid", "grid_back", "alpha_batched", "beta_batched", "gamma_push", "fuse_bu", "pack_bu",

@@ -28,7 +28,7 @@ EXPORTS = [
     "bcmi_graph_from_edges", "bcmi_gen_rmat", "bcmi_gen_pendant_trees",
     "bcmi_gen_block_chain", "bcmi_sample_sources", "bcmi_graph_free",
     "bcmi_create", "bcmi_destroy", "bcmi_set_param", "bcmi_get_param", "bcmi_upload_graph",
-    "bcmi_get_peel", "bcmi_get_labels", "bcmi_bc_reset", "bcmi_bc_run", "bcmi_get_bc", "bcmi_bc_run_static",
+    "bcmi_get_peel", "bcmi_get_labels", "bcmi_bc_reset", "bcmi_bc_run", "bcmi_get_bc", "bcmi_bc_run_static", "bcmi_batch_order",
     "bcmi_get_sweep_stats", "bcmi_get_sweep_times", "bcmi_get_kernel_time", "bcmi_sssp", "bcmi_msbfs",
     "bcmi_server_create", "bcmi_server_destroy", "bcmi_server_add_graph", "bcmi_server_submit",
     "bcmi_server_wait", "bcmi_server_stats", "bcmi_launch_plan",
@@ -106,6 +106,7 @@ def lib():
         L.bcmi_bc_run.argtypes = [vp, vp, i64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(RunStats)]
         L.bcmi_get_bc.argtypes = [vp, vp]
         L.bcmi_bc_run_static.argtypes = [vp, vp, i64, ctypes.POINTER(RunStats)]
+        L.bcmi_batch_order.argtypes = [vp, vp, i64, vp]
         L.bcmi_get_sweep_stats.argtypes = [vp, vp, vp, vp, vp]
         L.bcmi_get_sweep_times.argtypes = [vp, vp]
         L.bcmi_get_kernel_time.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
@@ -293,6 +294,13 @@ class Engine:
         st = RunStats()
         _check(lib().bcmi_bc_run_static(self._c, _p(src), src.shape[0], ctypes.byref(st)))
         return st.as_dict()
+
+    def batch_order(self, sources):
+        """The sources in the order a batched run groups them (degree-0 sources last)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        out = np.empty_like(src)
+        _check(lib().bcmi_batch_order(self._c, _p(src), src.shape[0], _p(out)))
+        return out
 
     def bc(self):
         out = np.zeros(self.n, np.float64)

@@ -388,6 +388,14 @@ int bcmi_bc_run_static(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, bcmi_r
   return rc;
 }
 
+int bcmi_batch_order(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int32_t* out) {
+  CHECK_CTX(c);
+  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+  if (nsrc < 0 || (nsrc > 0 && (!sources || !out))) return set_error(BCMI_ERR_ARG, "bad source list");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return batch_order(c->g, c->prm, sources, nsrc, out);
+}
+
 int bcmi_get_bc(bcmi_ctx* c, double* bc) {
   CHECK_CTX(c);
   if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");

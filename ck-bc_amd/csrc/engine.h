@@ -185,6 +185,10 @@ size_t static_scratch_need(const DevGraph& g);
 int bc_run_static(const DevGraph& g, double* bc, hipStream_t st, int device, const Params& prm,
                   const int32_t* sources, int64_t nsrc, StaticRun* out);
 
+// The sources in the order a batched run groups them (consecutive runs of
+// Params::width are its batches), degree-0 sources last; unpeeled graphs.
+int batch_order(const DevGraph& g, const Params& prm, const int32_t* src, int64_t nsrc, int32_t* out);
+
 // Debug sweeps (no BC accumulation).
 int sssp_debug(const DevGraph& g, Workspace* ws, const Params& prm, int32_t s, int32_t* depth,
                uint64_t* sigma, double* delta);

@@ -3554,6 +3554,71 @@ static int run_wg(const DevGraph& g, Workspace* ws, const Params& prm, const std
   return BCMI_OK;
 }
 
+// Order in which a batched run takes its sources (internal ids, degree > 0):
+// a permutation idx of cs, grouped by the Params::batching similarity key so
+// that consecutive runs of `width` sources share levels.
+static void batch_sort(const DevGraph& g, const Params& prm, const std::vector<int32_t>& cs,
+                       std::vector<size_t>& idx) {
+  idx.resize(cs.size());
+  for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+  const std::vector<uint64_t>& sg = g.h_sig;
+  const std::vector<int32_t>& tn = g.h_topnb;
+  const std::vector<int32_t>& tn2 = g.h_topnb2;
+  if (prm.batching == 3)
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+      uint32_t ka = ta < INT32_MAX ? sg[ta] : ~0u, kb = tb < INT32_MAX ? sg[tb] : ~0u;
+      if (ka != kb) return ka < kb;
+      if (ta != tb) return ta < tb;
+      return sg[cs[a]] < sg[cs[b]];
+    });
+  else if (prm.batching == 4)
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+      if (ta != tb) return ta < tb;
+      int32_t ua = tn2[cs[a]], ub = tn2[cs[b]];
+      if (ua != ub) return ua < ub;
+      return sg[cs[a]] < sg[cs[b]];
+    });
+  else if (prm.batching == 5)
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+      if (ta != tb) return ta < tb;
+      if (sg[cs[a]] != sg[cs[b]]) return sg[cs[a]] < sg[cs[b]];
+      return tn2[cs[a]] < tn2[cs[b]];
+    });
+  else if (prm.batching == 2)
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+      int32_t ta = tn[cs[a]], tb = tn[cs[b]];
+      if (ta != tb) return ta < tb;
+      return sg[cs[a]] < sg[cs[b]];
+    });
+  else if (prm.batching == 1)
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return sg[cs[a]] < sg[cs[b]]; });
+}
+
+int batch_order(const DevGraph& g, const Params& prm, const int32_t* src, int64_t nsrc, int32_t* out) {
+  if (g.heuristic) return set_error(BCMI_ERR_STATE, "batch order needs an unpeeled graph");
+  for (int64_t i = 0; i < nsrc; i++)
+    if (src[i] < 0 || src[i] >= g.n) return set_error(BCMI_ERR_RANGE, "source id out of range");
+  std::vector<int32_t> cs;
+  std::vector<int64_t> pos;
+  for (int64_t i = 0; i < nsrc; i++) {
+    const int32_t s = g.to_int(src[i]);
+    if (g.h_off[s + 1] > g.h_off[s]) { cs.push_back(s); pos.push_back(i); }
+  }
+  std::vector<size_t> idx;
+  batch_sort(g, prm, cs, idx);
+  int64_t k = 0;
+  for (size_t i : idx) out[k++] = src[pos[i]];
+  // sources of degree 0 are not swept: last, in the given order
+  for (int64_t i = 0; i < nsrc; i++) {
+    const int32_t s = g.to_int(src[i]);
+    if (g.h_off[s + 1] == g.h_off[s]) out[k++] = src[i];
+  }
+  return BCMI_OK;
+}
+
 int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* sources, int64_t nsrc, int mode,
            int src_flags, RunResult* out) {
   if (mode != ws->mode) return set_error(BCMI_ERR_STATE, "workspace was created for another mode");
@@ -3576,42 +3641,8 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   if (!plain && !wg) {
     // Batch sources with similar depth profiles together (BC is a sum over
     // sources, so the order only changes the rounding of the final sums).
-    std::vector<size_t> idx(cs.size());
-    for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
-    const std::vector<uint64_t>& sg = g.h_sig;
-    const std::vector<int32_t>& tn = g.h_topnb;
-    const std::vector<int32_t>& tn2 = g.h_topnb2;
-    if (prm.batching == 3)
-      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-        int32_t ta = tn[cs[a]], tb = tn[cs[b]];
-        uint32_t ka = ta < INT32_MAX ? sg[ta] : ~0u, kb = tb < INT32_MAX ? sg[tb] : ~0u;
-        if (ka != kb) return ka < kb;
-        if (ta != tb) return ta < tb;
-        return sg[cs[a]] < sg[cs[b]];
-      });
-    else if (prm.batching == 4)
-      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-        int32_t ta = tn[cs[a]], tb = tn[cs[b]];
-        if (ta != tb) return ta < tb;
-        int32_t ua = tn2[cs[a]], ub = tn2[cs[b]];
-        if (ua != ub) return ua < ub;
-        return sg[cs[a]] < sg[cs[b]];
-      });
-    else if (prm.batching == 5)
-      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-        int32_t ta = tn[cs[a]], tb = tn[cs[b]];
-        if (ta != tb) return ta < tb;
-        if (sg[cs[a]] != sg[cs[b]]) return sg[cs[a]] < sg[cs[b]];
-        return tn2[cs[a]] < tn2[cs[b]];
-      });
-    else if (prm.batching == 2)
-      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-        int32_t ta = tn[cs[a]], tb = tn[cs[b]];
-        if (ta != tb) return ta < tb;
-        return sg[cs[a]] < sg[cs[b]];
-      });
-    else if (prm.batching == 1)
-      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return sg[cs[a]] < sg[cs[b]]; });
+    std::vector<size_t> idx;
+    batch_sort(g, prm, cs, idx);
     std::vector<int32_t> cs2(cs.size());
     std::vector<double> cm2(cm.size());
     for (size_t i = 0; i < idx.size(); i++) { cs2[i] = cs[idx[i]]; cm2[i] = cm[idx[i]]; }

@@ -196,6 +196,16 @@ int bcmi_bc_run(bcmi_ctx *ctx, const int32_t *sources, int64_t nsrc, int mode, i
  * so the library runs them one at a time per GPU. */
 #define BCMI_STATIC_SCRATCH_BYTES (64ull << 20)
 int bcmi_bc_run_static(bcmi_ctx *ctx, const int32_t *sources, int64_t nsrc, bcmi_run_stats *st);
+/* The order in which bcmi_bc_run (batched mode) takes these sources: out
+ * receives the nsrc sources permuted by the "batching" similarity key,
+ * computed on the host from the uploaded graph, with the sources of degree 0
+ * (not swept) last; consecutive runs of "width" entries are the batches of
+ * one batched run over the whole list.  Passing such runs to later calls with
+ * "batching" = 0 reproduces those batches, so one workload can be split over
+ * several calls or devices, whole batches each, without losing batch quality.
+ * Unpeeled graphs only (BCMI_ERR_STATE).  No reference counterpart (its
+ * source loop takes startv, startv+1, ... in turn, c:1017). */
+int bcmi_batch_order(bcmi_ctx *ctx, const int32_t *sources, int64_t nsrc, int32_t *out);
 /* Copies the BC accumulator to the host (replaces get_bc cu:1039-1041;
  * double instead of float, not halved). */
 int bcmi_get_bc(bcmi_ctx *ctx, double *bc);

@@ -367,3 +367,37 @@ def test_batched_counts_past_32_bits(narrow):
         e.run(src, mode=bcmi.MODE_BATCHED)
         assert bcmi.rel_err(e.bc(), ref) <= BC_TOL
     e.close()
+
+
+def test_batch_order_reproduces_the_batches(engine):
+    """bcmi_batch_order lists the sources as a batched run groups them
+    (degree-0 sources last): run in that order with "batching" 0 they form
+    the same batches (same (vertex, level) pairs) and give the same BC, also
+    when split over two calls of whole batches."""
+    g = bcmi.Graph.rmat(12, 16, seed=5)
+    iso = np.flatnonzero(g.degrees() == 0)[:5].astype(np.int32)
+    src = np.concatenate([g.sample_sources(300, seed=6), iso]).astype(np.int32)
+    engine.upload(g)
+    order = engine.batch_order(src)
+    assert sorted(order.tolist()) == sorted(src.tolist())
+    assert set(order[-len(iso):].tolist()) == set(iso.tolist())
+    engine.reset()
+    st = engine.run(src, mode=bcmi.MODE_BATCHED)
+    full = engine.bc()
+    engine.set_param("batching", 0)
+    try:
+        engine.reset()
+        st2 = engine.run(order[:300], mode=bcmi.MODE_BATCHED)
+        assert st2["sum_pairs"] == st["sum_pairs"] and st2["batches"] == st["batches"]
+        assert bcmi.rel_err(engine.bc(), full) <= 1e-9
+        engine.reset()
+        engine.run(order[:128], mode=bcmi.MODE_BATCHED)
+        engine.run(order[128:300], mode=bcmi.MODE_BATCHED)
+        assert bcmi.rel_err(engine.bc(), full) <= 1e-9
+        assert bcmi.rel_err(full, O.brandes(g.off, g.adj, src, nthreads=4)) <= 1e-9
+    finally:
+        engine.set_param("batching", 4)
+    h = bcmi.Graph.rmat(9, 8, seed=7)
+    engine.upload(h, heuristic=bcmi.H_PEEL)
+    with pytest.raises(bcmi.BcmiError):
+        engine.batch_order(h.sample_sources(8, seed=1))
