@@ -685,10 +685,18 @@ __device__ __forceinline__ void k_plain_bu_body(GA g, PA p) {
       cnt = (int)wave_sum_ll(cnt);
       bool won = false;
       if (cnt > 0 && lane == 0) {
-        int old = atomicCAS(&p.depth[v], -1, d + 1);
-        won = (old == -1);
-        u64 o = atomicAdd(&p.sigma[v], acc);
-        if (o + acc < o) ovf = 1;
+        if (g.off[v + 1] - g.off[v] <= g.chunk) {
+          // v's only record: the sole writer of its depth and path count in
+          // this step (unvisited: depth -1, count 0), so plain stores claim it
+          p.depth[v] = d + 1;
+          p.sigma[v] = acc;
+          won = true;
+        } else {
+          int old = atomicCAS(&p.depth[v], -1, d + 1);
+          won = (old == -1);
+          u64 o = atomicAdd(&p.sigma[v], acc);
+          if (o + acc < o) ovf = 1;
+        }
         if (won) p.delta[v] = g.w ? (double)g.w[v] : 0.0;
         acnt += cnt;
       }
@@ -1129,13 +1137,19 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       bool won = false;
       int64_t dg = 0;
       if (acc && lane == 0) {
-        if ((nxt[v] & acc) != acc) {
+        if (!split) {
+          // v's only record: no other writer of its next-level mask in this
+          // step, and the mask is zero at the step's start (k_ms_sigma cleared
+          // the array), so a plain store claims it without a returning atomic
+          nxt[v] = acc;
+          won = true;
+        } else if ((nxt[v] & acc) != acc) {
           u64 old = atomicOr(&nxt[v], acc);
           won = (old == 0);
-          if (won) {
-            dg = deg;
-            if (g.flags & KF_GATE) lb_set(nlb, v);
-          }
+        }
+        if (won) {
+          dg = deg;
+          if (g.flags & KF_GATE) lb_set(nlb, v);
         }
       }
       if (fr && ((acc >> lane) & 1)) {
@@ -1252,14 +1266,11 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         bool won = false;
         int64_t dg = 0;
         if (sl == 0 && acc) {
-          if ((nxt[v] & acc) != acc) {
-            u64 old = atomicOr(&nxt[v], acc);
-            won = (old == 0);
-            if (won) {
-              dg = dgv;
-              if (g.flags & KF_GATE) lb_set(nlb, v);
-            }
-          }
+          // degree <= PACK_DEG: v's only record, a plain store claims it (as above)
+          nxt[v] = acc;
+          won = true;
+          dg = dgv;
+          if (g.flags & KF_GATE) lb_set(nlb, v);
         }
         if (fuse) {
 #pragma unroll
