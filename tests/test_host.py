@@ -204,9 +204,10 @@ def test_tuned_table_names_known_parameters():
     import autotune
     table = autotune.load()
     assert {"default", "rmat-s18-ef16", "rmat-s22-ef16", "rmat-s25-ef16"} <= set(table)
-    notes = {"batched_gteps", "plain_gteps", "tuned_on"}
+    notes = set(autotune.NOTES)
     applied = {"chunk", "block", "grid", "grid_back", "grid_plain", "alpha_batched", "beta_batched", "gamma_push",
                "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width"}
+    assert set(autotune.APPLIED) == applied
 
     class Rec:
         def __init__(self):
@@ -260,3 +261,25 @@ def test_read_tiny_files_many_threads(tmp_path):
     got = json.loads(r.stdout.strip().splitlines()[-1])
     for name, (_, want) in cases.items():
         assert got[str(tmp_path / name)] == want, name
+
+
+def test_ck_run_commands_use_the_tuned_shapes():
+    """The CK run commands that pass launch shapes (-T key=value) pass the
+    autotuned table's values for their graph's R-MAT scale (-n = 2^scale)."""
+    import autotune
+    meta = json.load(open(os.path.join(ROOT, "ck-bc_amd", "program", "bc-hip", ".cm", "meta.json")))
+    table = autotune.load()
+    seen = 0
+    for name, cmd in meta["run_cmds"].items():
+        line = cmd["run_time"]["run_cmd_main"]
+        tun = re.findall(r"-T (\w+)=(\d+)", line)
+        if not tun:
+            continue
+        n = int(re.search(r"-n (\d+)", line).group(1))
+        scale = n.bit_length() - 1
+        assert n == 1 << scale, name
+        entry = table[autotune.graph_key("rmat", scale=scale, edgefactor=16)]
+        for k, v in tun:
+            assert k in autotune.APPLIED and int(v) == entry[k], (name, k, v, entry[k])
+        seen += 1
+    assert seen >= 2
