@@ -296,16 +296,30 @@ __device__ __forceinline__ void sg_st(u64* sg, bool nar, int64_t i, u64 x) {
   if (nar) reinterpret_cast<uint32_t*>(sg)[i] = (uint32_t)x;
   else sg[i] = x;
 }
-// One gather of an unrolled group.  32-bit rows load unconditionally (an
-// unused lane reads element 0) so the group's loads stay in flight together
-// instead of each waiting behind a branch.
+// True on the lanes whose bit is set in the wave-uniform mask m: the mask
+// becomes the exec mask as it is, with no per-lane shift and compare.
+__device__ __forceinline__ bool in_mask(u64 m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+// One gather of an unrolled group: the lanes of the wave-uniform mask m read
+// their element of row `row` (exec = m), the others get 0.  The value keeps
+// the row's width, so the load writes its destination register directly and
+// the loads of a group stay in flight together until the first use.
+template <bool NAR> struct RowT { using T = u64; };
+template <> struct RowT<true> { using T = uint32_t; };
 template <bool NAR>
-__device__ __forceinline__ u64 sg_gather(const u64* sg, bool use, int64_t i) {
-  if (NAR) {
-    const uint32_t x = reinterpret_cast<const uint32_t*>(sg)[use ? i : 0];
-    return use ? (u64)x : 0;
-  }
-  return use ? sg[i] : 0;
+__device__ __forceinline__ typename RowT<NAR>::T sg_gather(const u64* sg, u64 m, int32_t row) {
+  using T = typename RowT<NAR>::T;
+  T x = 0;
+  if (in_mask(m)) x = reinterpret_cast<const T*>(sg)[(int64_t)row * 64 + lane_id()];
+  return x;
+}
+// Adds a 32-bit term to a 64-bit count kept as two words (lo, hi).  The term
+// stays 32 bits wide up to the add, so the gather before it loads straight
+// into its register (a widened value would have to wait for the load inside
+// the gather's branch).
+__device__ __forceinline__ void add_w(uint32_t& lo, uint32_t& hi, uint32_t x) {
+  uint32_t t;
+  hi += __builtin_add_overflow(lo, x, &t);
+  lo = t;
 }
 // stores an exact count; true when it does not fit the row
 __device__ __forceinline__ bool sg_put(u64* sg, bool nar, int64_t i, u64 x) {
@@ -1090,6 +1104,8 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       const bool fr = fuse && (!split || b.fuse > 1);
       u64 acc = 0, sacc = 0;
       double facc = 0.0;
+      uint32_t slo = 0, shi = 0;  // 32-bit rows: the exact sum as two words
+      uint32_t cnt = 0;  // this record's parents per source (at most its arcs)
       for (int64_t eb = e0; eb < e1; eb += 64) {
         int64_t e = eb + lane;
         int32_t u = 0;
@@ -1111,18 +1127,23 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
             q[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
             if (okq[t]) P &= P - 1;
           }
-          u64 x[UNR_BU];
+          typename RowT<NAR>::T x[UNR_BU];
 #pragma unroll
           for (int t = 0; t < UNR_BU; t++) {
             const int32_t uq = rl32(u, q[t]);
-            const u64 nq = rl64(need, q[t]);
-            const bool use = okq[t] && ((nq >> lane) & 1);
-            x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
-            ca += use;
+            const u64 nq = okq[t] ? rl64(need, q[t]) : 0ull;
+            x[t] = sg_gather<NAR>(b.sigma, nq, uq);
+            cnt += in_mask(nq);
           }
-          if (scaled) {
+          // the lanes outside a gather's mask add 0 (exact in both forms;
+          // scaled counts always use 64-bit rows)
+          if (scaled && !NAR) {
 #pragma unroll
             for (int t = 0; t < UNR_BU; t++) facc = facc + __longlong_as_double((long long)x[t]);
+          } else if constexpr (NAR) {
+            // 32-bit terms: a 64-bit sum of fewer than 2^32 of them cannot wrap
+#pragma unroll
+            for (int t = 0; t < UNR_BU; t++) add_w(slo, shi, x[t]);
           } else {
 #pragma unroll
             for (int t = 0; t < UNR_BU; t++) {
@@ -1133,6 +1154,8 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
           }
         }
       }
+      ca += cnt;
+      if (NAR) sacc = ((u64)shi << 32) | slo;
       acc = wave_or64(acc) & ~sv;
       bool won = false;
       int64_t dg = 0;
@@ -1223,8 +1246,9 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
 #pragma unroll
         for (int o = PACK_DEG / 2; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
         u64 sq[PACK_W];  // per sub-group: exact sum, or FP64 bits when scaled
+        uint32_t wl[PACK_W], wh[PACK_W];  // 32-bit rows: the exact sums as two words
 #pragma unroll
-        for (int q = 0; q < PACK_W; q++) sq[q] = 0;
+        for (int q = 0; q < PACK_W; q++) sq[q] = 0, wl[q] = 0, wh[q] = 0;
         if (fuse) {
           u64 P = __ballot(need != 0);
           while (P) {
@@ -1236,14 +1260,13 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
               qa[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
               if (okq[t]) P &= P - 1;
             }
-            u64 x[UNR_PK];
+            typename RowT<NAR>::T x[UNR_PK];
 #pragma unroll
             for (int t = 0; t < UNR_PK; t++) {
               const int32_t uq = rl32(u, qa[t]);
-              const u64 nq = rl64(need, qa[t]);
-              const bool use = okq[t] && ((nq >> lane) & 1);
-              x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
-              ca += use;
+              const u64 nq = okq[t] ? rl64(need, qa[t]) : 0ull;
+              x[t] = sg_gather<NAR>(b.sigma, nq, uq);
+              ca += in_mask(nq);
             }
 #pragma unroll
             for (int t = 0; t < UNR_PK; t++) {
@@ -1251,7 +1274,9 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
 #pragma unroll
               for (int q = 0; q < PACK_W; q++) {
                 if (gq != q || !okq[t]) continue;
-                if (scaled) {
+                if constexpr (NAR) {
+                  add_w(wl[q], wh[q], x[t]);  // 32-bit terms cannot wrap a 64-bit sum
+                } else if (scaled) {
                   sq[q] = (u64)__double_as_longlong(__longlong_as_double((long long)sq[q]) +
                                                     __longlong_as_double((long long)x[t]));
                 } else {
@@ -1262,6 +1287,10 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
               }
             }
           }
+        }
+        if (NAR) {
+#pragma unroll
+          for (int q = 0; q < PACK_W; q++) sq[q] = ((u64)wh[q] << 32) | wl[q];
         }
         bool won = false;
         int64_t dg = 0;
@@ -1407,6 +1436,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
     }
     const bool mine = (N >> lane) & 1;
     u64 acc = 0;
+    uint32_t alo = 0, ahi = 0;  // 32-bit rows: the exact sum as two words
     double accf = 0.0;
     for (int64_t eb = e0; eb < e1; eb += 64) {
       int64_t e = eb + lane;
@@ -1428,18 +1458,20 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
           q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
           if (ok[t]) P &= P - 1;
         }
-        u64 x[UNR];
+        typename RowT<NAR>::T x[UNR];
 #pragma unroll
         for (int t = 0; t < UNR; t++) {
-          int32_t uq = rl32(u, q[t]);
-          u64 fq = rl64(F, q[t]);
-          bool use = ok[t] && ((fq >> lane) & 1);
-          x[t] = sg_gather<NAR>(b.sigma, use, (int64_t)uq * 64 + lane);
-          ca += use;
+          const int32_t uq = rl32(u, q[t]);
+          const u64 fq = ok[t] ? rl64(F, q[t]) : 0ull;
+          x[t] = sg_gather<NAR>(b.sigma, fq, uq);
+          ca += in_mask(fq);
         }
-        if (scaled) {
+        if (scaled && !NAR) {
 #pragma unroll
           for (int t = 0; t < UNR; t++) accf = accf + __longlong_as_double((long long)x[t]);
+        } else if constexpr (NAR) {
+#pragma unroll
+          for (int t = 0; t < UNR; t++) add_w(alo, ahi, x[t]);
         } else {
 #pragma unroll
           for (int t = 0; t < UNR; t++) {
@@ -1450,6 +1482,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
         }
       }
     }
+    if (NAR) acc = ((u64)ahi << 32) | alo;
     u64* row = b.sigma + (int64_t)v * 64 + lane;
     if (done_by_bu) {
       // row written by k_ms_bu
@@ -1621,17 +1654,17 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
         if (ok[t]) P &= P - 1;
       }
       double x[UNR];
-      bool use[UNR];
 #pragma unroll
       for (int t = 0; t < UNR; t++) {
-        int32_t zq = rl32(z, q[t]);
-        u64 mq = rl64(M, q[t]);
-        use[t] = ok[t] && ((mq >> lane) & 1);
-        x[t] = use[t] ? b.coef[(int64_t)zq * 64 + lane] : 0.0;
+        const int32_t zq = rl32(z, q[t]);
+        const u64 mq = ok[t] ? rl64(M, q[t]) : 0ull;
+        x[t] = 0.0;
+        if (in_mask(mq)) x[t] = b.coef[(int64_t)zq * 64 + lane];
       }
+      // a lane outside a successor's mask adds 0 * sv = +0 (sv is positive
+      // and finite, acc never -0), which leaves acc unchanged
 #pragma unroll
-      for (int t = 0; t < UNR; t++)
-        if (use[t]) acc = acc + x[t] * sv;
+      for (int t = 0; t < UNR; t++) acc = acc + x[t] * sv;
     }
   }
   double* crow = b.coef + (int64_t)v * 64 + lane;
