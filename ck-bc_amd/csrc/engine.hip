@@ -79,6 +79,35 @@ __device__ __forceinline__ u64 rl64(u64 x, int l) {
 }
 __device__ __forceinline__ int rl32(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
 
+// Takes the next U set bits (lanes) of the wave-uniform mask P, in lane
+// order, and hands them to body(q, ok).  While P holds at least U bits the
+// slots are all valid and need no per-slot test (FULL: a straight run of
+// find-first-bit and bit-clear); the last, partial group tests each slot.
+template <int U, bool FULL, class Body>
+__device__ __forceinline__ void take_lanes(u64& P, Body&& body) {
+  int q[U];
+  bool ok[U];
+#pragma unroll
+  for (int t = 0; t < U; t++) {
+    if (FULL) {
+      ok[t] = true;
+      q[t] = __builtin_ctzll(P);
+      P &= ~(1ull << q[t]);
+    } else {
+      ok[t] = P != 0;
+      q[t] = ok[t] ? __builtin_ctzll(P) : 0;
+      if (ok[t]) P &= P - 1;
+    }
+  }
+  body(q, ok);
+}
+// Runs body over every set bit of P, U at a time.
+template <int U, class Body>
+__device__ __forceinline__ void for_lanes(u64 P, Body&& body) {
+  while (__popcll(P) >= U) take_lanes<U, true>(P, body);
+  if (P) take_lanes<U, false>(P, body);
+}
+
 __device__ __forceinline__ int wave_excl_scan(int x, int* total) {
   int lane = lane_id();
   int inc = x;
@@ -1117,16 +1146,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         }
         if (!fr) continue;
         const u64 need = F & ~sv;  // u is a parent of v for these sources
-        u64 P = __ballot(need != 0);
-        while (P) {
-          int q[UNR_BU];
-          bool okq[UNR_BU];
-#pragma unroll
-          for (int t = 0; t < UNR_BU; t++) {
-            okq[t] = P != 0;
-            q[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
-            if (okq[t]) P &= P - 1;
-          }
+        for_lanes<UNR_BU>(__ballot(need != 0), [&](const int (&q)[UNR_BU], const bool (&okq)[UNR_BU]) {
           typename RowT<NAR>::T x[UNR_BU];
 #pragma unroll
           for (int t = 0; t < UNR_BU; t++) {
@@ -1152,7 +1172,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
               sacc = s2;
             }
           }
-        }
+        });
       }
       ca += cnt;
       if (NAR) sacc = ((u64)shi << 32) | slo;
@@ -1446,18 +1466,9 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
         u = adj_at(g, g.tadj, e);
         if (!(g.flags & KF_GATE) || lb_test(clb, u)) F = cur[u] & N;
       }
-      u64 P = __ballot(F != 0);
       // Parents UNR at a time: the row loads are issued before any is
       // consumed, and the sums still follow adjacency order.
-      while (P) {
-        int q[UNR];
-        bool ok[UNR];
-#pragma unroll
-        for (int t = 0; t < UNR; t++) {
-          ok[t] = P != 0;
-          q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
-          if (ok[t]) P &= P - 1;
-        }
+      for_lanes<UNR>(__ballot(F != 0), [&](const int (&q)[UNR], const bool (&ok)[UNR]) {
         typename RowT<NAR>::T x[UNR];
 #pragma unroll
         for (int t = 0; t < UNR; t++) {
@@ -1480,7 +1491,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
             acc = s;
           }
         }
-      }
+      });
     }
     if (NAR) acc = ((u64)ahi << 32) | alo;
     u64* row = b.sigma + (int64_t)v * 64 + lane;
@@ -1641,18 +1652,9 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
       z = adj_at(g, g.adj, e);
       if (!(g.flags & KF_GATE) || lb_test(nlb, z)) M = lmn[z] & L;
     }
-    u64 P = __ballot(M != 0);
     // Successors UNR at a time (loads first, then the sums in adjacency
     // order; FP contraction is off, so each step rounds like the oracle).
-    while (P) {
-      int q[UNR];
-      bool ok[UNR];
-#pragma unroll
-      for (int t = 0; t < UNR; t++) {
-        ok[t] = P != 0;
-        q[t] = ok[t] ? __ffsll((long long)P) - 1 : 0;
-        if (ok[t]) P &= P - 1;
-      }
+    for_lanes<UNR>(__ballot(M != 0), [&](const int (&q)[UNR], const bool (&ok)[UNR]) {
       double x[UNR];
 #pragma unroll
       for (int t = 0; t < UNR; t++) {
@@ -1665,7 +1667,7 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
       // and finite, acc never -0), which leaves acc unchanged
 #pragma unroll
       for (int t = 0; t < UNR; t++) acc = acc + x[t] * sv;
-    }
+    });
   }
   double* crow = b.coef + (int64_t)v * 64 + lane;
   if (nch > 1) {
