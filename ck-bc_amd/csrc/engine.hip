@@ -1348,7 +1348,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_ms_bu(GA g, BA b, KT* kt) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ms_bu(GA g, BA b, KT* kt) {
   kt_enter(kt);
   if (b.ctl->narrow) k_ms_bu_body<true, false>(g, b);
   else k_ms_bu_body<false, false>(g, b);
