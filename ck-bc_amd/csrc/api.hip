@@ -140,6 +140,8 @@ static int param_set(Params& p, const std::string& k, int64_t v, bool have_graph
     p.reorder = v ? 1 : 0;
   } else if (k == "trace") {
     p.trace = v ? 1 : 0;
+  } else if (k == "graphs") {
+    p.graphs = v ? 1 : 0;
   } else if (k == "ktime") {
     if (v < 0 || v > 7) return set_error(BCMI_ERR_ARG, "ktime must be in [0,7]");
     p.ktime = (int)v;
@@ -238,6 +240,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "width") *v = p.width;
   else if (k == "ktime") *v = p.ktime;
   else if (k == "trace") *v = p.trace;
+  else if (k == "graphs") *v = p.graphs;
   else if (k == "reorder") *v = p.reorder;
   else if (k == "batching") *v = p.batching;
   else if (k == "kflags") *v = p.kflags;

@@ -1070,7 +1070,9 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 // kernel skips its parent scan: a vertex held in one record stores its sum,
 // the records of a split vertex (fuse_bu 2) add theirs atomically into its
 // row, whose columns are zero until the source reaches it.
-#define UNR_BU 8
+#ifndef UNR_BU
+#define UNR_BU 12  // parent gathers in flight per wave in the bottom-up kernel
+#endif
 #define UNR_PK 4  // parent gathers in flight per wave in the packed bottom-up kernel
 #define PACK_DEG 16  // packed bottom-up: vertices of degree <= PACK_DEG ...
 #define PACK_W 4     // ... PACK_W to a wave (PACK_DEG * PACK_W = 64)
@@ -1758,7 +1760,10 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
 
 // 8 waves per SIMD (64 VGPRs; a few bytes spill outside the gather loop): more
 // row gathers in flight per CU than 7 waves with no spill.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_ms_back(GA g, BA b, KT* kt) {
+#ifndef BACK_WAVES
+#define BACK_WAVES 8
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BACK_WAVES))) k_ms_back(GA g, BA b, KT* kt) {
   kt_enter(kt);
   if (b.ctl->narrow) k_ms_back_body<true>(g, b);
   else k_ms_back_body<false>(g, b);
@@ -3071,51 +3076,60 @@ std::vector<StepNode> step_plan(const Params& prm, bool plain, int graph) {
   return v;
 }
 
-// Captures `steps` forward levels and `steps` backward levels into graphs
-// (kernel nodes only), each step launched as step_plan lists it.
-static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
-  ALLOC_LOCK;
-  free_graphs(ws);
+// Launches `steps` levels of graph `which` (0 forward, 1 backward, 2 and 3
+// their small-grid forms) on the workspace's stream, each step as step_plan
+// lists it.  Captured into graphs by capture(); launched as it is when the
+// level loop runs without graphs.
+static void launch_steps(const DevGraph& g, Workspace* ws, const Params& prm, int which) {
   const GA ga = make_ga(g, prm);
   const PA pa = make_pa(ws, prm);
   const BA ba = make_ba(ws, prm);
   hipStream_t st = ws->st;
   const bool plain = ws->mode == BCMI_MODE_PLAIN;
   KT* const T = ws->kt;
+  const std::vector<StepNode> plan = step_plan(prm, plain, which);
+  for (int s = 0; s < prm.steps; s++) {
+    for (const StepNode& nd : plan) {
+      KT* const kt = nd.timed ? T : nullptr;
+      const dim3 G(nd.grid), B(nd.block);
+      switch (nd.kernel) {
+        case SK_PLAIN_DECIDE: hipLaunchKernelGGL(k_plain_decide, G, B, 0, st, ga, pa, (int)nd.aux); break;
+        case SK_PLAIN_TD: hipLaunchKernelGGL(k_plain_td, G, B, 0, st, ga, pa, kt); break;
+        case SK_PLAIN_BU: hipLaunchKernelGGL(k_plain_bu, G, B, 0, st, ga, pa, kt); break;
+        case SK_BDECIDE: hipLaunchKernelGGL(k_bdecide, G, B, 0, st, pa.ctl); break;
+        case SK_PLAIN_BACK: hipLaunchKernelGGL(k_plain_back, G, B, 0, st, ga, pa, kt); break;
+        case SK_PLAIN_BFIN: hipLaunchKernelGGL(k_plain_bfin, G, B, 0, st, ga, pa); break;
+        case SK_MS_DECIDE: hipLaunchKernelGGL(k_ms_decide, G, B, 0, st, ga, ba, (int)nd.aux); break;
+        case SK_MS_TD: hipLaunchKernelGGL(k_ms_td, G, B, 0, st, ga, ba, kt); break;
+        case SK_MS_BU: hipLaunchKernelGGL(k_ms_bu, G, B, 0, st, ga, ba, kt); break;
+        case SK_MS_BU_PK: hipLaunchKernelGGL(k_ms_bu_pk, G, B, 0, st, ga, ba, kt); break;
+        case SK_MS_SIGMA: hipLaunchKernelGGL(k_ms_sigma, G, B, 0, st, ga, ba, kt); break;
+        case SK_MS_PUSH: hipLaunchKernelGGL(k_ms_push, G, B, 0, st, ga, ba); break;
+        case SK_MS_BDECIDE: hipLaunchKernelGGL(k_ms_bdecide, G, B, 0, st, ga, ba); break;
+        case SK_MS_BACK: hipLaunchKernelGGL(k_ms_back, G, B, 0, st, ga, ba, kt); break;
+        case SK_MS_BPUSH: hipLaunchKernelGGL(k_ms_bpush, G, B, 0, st, ga, ba, kt); break;
+        case SK_MS_BFIN: hipLaunchKernelGGL(k_ms_bfin, G, B, 0, st, ga, ba, kt); break;
+        case SK_KT_ACCUM: hipLaunchKernelGGL(k_kt_accum, G, B, 0, st, T, nd.aux); break;
+        case SK_KT_FOLD: hipLaunchKernelGGL(k_kt_fold, G, B, 0, st, T); break;
+        default: break;
+      }
+    }
+  }
+}
+
+// Captures `steps` forward levels and `steps` backward levels into graphs
+// (kernel nodes only).
+static int capture(const DevGraph& g, Workspace* ws, const Params& prm) {
+  ALLOC_LOCK;
+  free_graphs(ws);
+  hipStream_t st = ws->st;
   // Four graphs: {large grid, small grid} x {forward, backward}.  The small
   // one serves levels with few vertices (long chains): its launches are cheap
   // and it never considers bottom-up (which scans every vertex).
   for (int which = 0; which < 4; which++) {
-    const std::vector<StepNode> plan = step_plan(prm, plain, which);
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < prm.steps; s++) {
-      for (const StepNode& nd : plan) {
-        KT* const kt = nd.timed ? T : nullptr;
-        const dim3 G(nd.grid), B(nd.block);
-        switch (nd.kernel) {
-          case SK_PLAIN_DECIDE: hipLaunchKernelGGL(k_plain_decide, G, B, 0, st, ga, pa, (int)nd.aux); break;
-          case SK_PLAIN_TD: hipLaunchKernelGGL(k_plain_td, G, B, 0, st, ga, pa, kt); break;
-          case SK_PLAIN_BU: hipLaunchKernelGGL(k_plain_bu, G, B, 0, st, ga, pa, kt); break;
-          case SK_BDECIDE: hipLaunchKernelGGL(k_bdecide, G, B, 0, st, pa.ctl); break;
-          case SK_PLAIN_BACK: hipLaunchKernelGGL(k_plain_back, G, B, 0, st, ga, pa, kt); break;
-          case SK_PLAIN_BFIN: hipLaunchKernelGGL(k_plain_bfin, G, B, 0, st, ga, pa); break;
-          case SK_MS_DECIDE: hipLaunchKernelGGL(k_ms_decide, G, B, 0, st, ga, ba, (int)nd.aux); break;
-          case SK_MS_TD: hipLaunchKernelGGL(k_ms_td, G, B, 0, st, ga, ba, kt); break;
-          case SK_MS_BU: hipLaunchKernelGGL(k_ms_bu, G, B, 0, st, ga, ba, kt); break;
-          case SK_MS_BU_PK: hipLaunchKernelGGL(k_ms_bu_pk, G, B, 0, st, ga, ba, kt); break;
-          case SK_MS_SIGMA: hipLaunchKernelGGL(k_ms_sigma, G, B, 0, st, ga, ba, kt); break;
-          case SK_MS_PUSH: hipLaunchKernelGGL(k_ms_push, G, B, 0, st, ga, ba); break;
-          case SK_MS_BDECIDE: hipLaunchKernelGGL(k_ms_bdecide, G, B, 0, st, ga, ba); break;
-          case SK_MS_BACK: hipLaunchKernelGGL(k_ms_back, G, B, 0, st, ga, ba, kt); break;
-          case SK_MS_BPUSH: hipLaunchKernelGGL(k_ms_bpush, G, B, 0, st, ga, ba, kt); break;
-          case SK_MS_BFIN: hipLaunchKernelGGL(k_ms_bfin, G, B, 0, st, ga, ba, kt); break;
-          case SK_KT_ACCUM: hipLaunchKernelGGL(k_kt_accum, G, B, 0, st, T, nd.aux); break;
-          case SK_KT_FOLD: hipLaunchKernelGGL(k_kt_fold, G, B, 0, st, T); break;
-          default: break;
-        }
-      }
-    }
+    launch_steps(g, ws, prm, which);
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) return set_error(BCMI_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
     hipGraphExec_t ex;
@@ -3140,12 +3154,26 @@ static bool same_launch(const Params& a, const Params& b) {
          a.kflags == b.kflags && a.sgrid == b.sgrid;
 }
 
+// One replay of graph `which` (0 forward, 1 backward, 2 and 3 their
+// small-grid forms): the captured graph, or with prm.graphs == 0 the same
+// kernels launched one by one.
+static int replay(const DevGraph& g, Workspace* ws, const Params& prm, int which) {
+  if (!prm.graphs) {
+    launch_steps(g, ws, prm, which);
+    HIP_TRY(hipGetLastError());
+    return BCMI_OK;
+  }
+  hipGraphExec_t ex = which == 0 ? ws->gfwd : which == 1 ? ws->gbwd : which == 2 ? ws->gfwd_s : ws->gbwd_s;
+  HIP_TRY(hipGraphLaunch(ex, ws->st));
+  return BCMI_OK;
+}
+
 // Runs one sweep (plain: srcbuf[0]; batched: srcbuf[0..k)) whose inputs are
 // already in the workspace buffers.  keep != 0 skips the cleanup (debug
 // readback), the caller then calls sweep_cleanup.
 static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, bool keep, int* levels, int scaled = 0,
                  int narrow = 0) {
-  if (!ws->gfwd || ws->cap_graph != &g || !same_launch(ws->cap_prm, prm)) TRY(capture(g, ws, prm));
+  if (prm.graphs && (!ws->gfwd || ws->cap_graph != &g || !same_launch(ws->cap_prm, prm))) TRY(capture(g, ws, prm));
   GA ga = make_ga(g, prm);
   hipStream_t st = ws->st;
   for (int attempt = 0;; attempt++) {
@@ -3167,7 +3195,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       HIP_TRY(hipStreamSynchronize(st));
       for (int it = 0; it < 100000; it++) {
         double t0 = now_ms();
-        HIP_TRY(hipGraphLaunch(ws->gfwd, st));
+        TRY(replay(g, ws, prm, 0));
         HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         const Ctl& c = *ws->hctl;
@@ -3178,21 +3206,21 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       }
       for (int it = 0; it < 100000; it++) {
         double t0 = now_ms();
-        HIP_TRY(hipGraphLaunch(ws->gbwd, st));
+        TRY(replay(g, ws, prm, 1));
         HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         fprintf(stderr, "[trace] bwd level %d %.3f ms\n", ws->hctl->blevel, now_ms() - t0);
         if (ws->hctl->bdone) break;
       }
     }
-    HIP_TRY(hipGraphLaunch(ws->gfwd, st));
-    HIP_TRY(hipGraphLaunch(ws->gbwd, st));
+    TRY(replay(g, ws, prm, 0));
+    TRY(replay(g, ws, prm, 1));
     HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     while (!ws->hctl->fdone && !ws->hctl->overflow) {
       // small levels (long chains) replay the small-grid graph
       int64_t front = std::max<int64_t>(ws->hctl->verts_cur, ws->hctl->verts_next);
-      HIP_TRY(hipGraphLaunch(front < prm.sparse_verts ? ws->gfwd_s : ws->gfwd, st));
+      TRY(replay(g, ws, prm, front < prm.sparse_verts ? 2 : 0));
       HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
@@ -3224,7 +3252,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       } else {
         TRY(alloc_records(ws, ws->cap * 4));
       }
-      TRY(capture(g, ws, prm));
+      if (prm.graphs) TRY(capture(g, ws, prm));
       continue;
     }
     if (!ws->hctl->bdone) {
@@ -3238,7 +3266,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
         int bot = std::max(0, top - prm.steps + 1);
         int64_t recs = lo[std::min(top + 1, last + 1)] - lo[bot];
         bool sparse = recs < prm.sparse_verts * (int64_t)(top - bot + 1);
-        HIP_TRY(hipGraphLaunch(sparse ? ws->gbwd_s : ws->gbwd, st));
+        TRY(replay(g, ws, prm, sparse ? 3 : 1));
         HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
       }

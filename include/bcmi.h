@@ -154,7 +154,9 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * per source: deep graphs, whose hub BFS depth reaches "wg_depth", run
  * hundreds of sources at once, one sweep per workgroup, within "wg_mem"
  * bytes of per-source state and at most "wg_slots" sources in flight),
- * "trace". */
+ * "trace", "graphs" (1, the default: the level loop replays captured
+ * graphs; 0: it launches the same kernels one by one, for profilers and
+ * other tools that do not follow graph launches). */
 int bcmi_set_param(bcmi_ctx *ctx, const char *key, int64_t value);
 int bcmi_get_param(bcmi_ctx *ctx, const char *key, int64_t *value);
 

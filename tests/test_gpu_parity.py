@@ -243,6 +243,33 @@ def test_kernel_timer(engine):
     engine.set_param("ktime", 0)
 
 
+def test_level_loop_without_graphs():
+    """graphs = 0 launches the captured graphs' kernels one by one: depths
+    and path counts equal the captured run's bit for bit and dependencies
+    within 1e-12 (split hubs add their partial sums atomically), one batch and
+    one plain sweep; BC matches the oracle in both modes, and the device timer
+    still sees the timed kernel."""
+    g = bcmi.Graph.rmat(12, 16, seed=3)
+    src = g.sample_sources(160, seed=5)
+    ref = O.brandes(g.off, g.adj, src, nthreads=4)
+    out = {}
+    for graphs in (1, 0):
+        e = bcmi.Engine(graphs=graphs, ktime=2)
+        assert e.get_param("graphs") == graphs
+        e.upload(g)
+        out[graphs] = (e.msbfs(src[:64]), e.sssp(int(src[0])))
+        for mode in (bcmi.MODE_PLAIN, bcmi.MODE_BATCHED):
+            e.reset()
+            e.run(src, mode=mode)
+            assert bcmi.rel_err(e.bc(), ref) <= BC_TOL, (graphs, mode)
+        ms, launches = e.kernel_time()
+        assert launches > 0 and ms > 0, graphs
+        e.close()
+    for run1, run0 in zip(out[1], out[0]):
+        assert np.array_equal(run1[0], run0[0]) and np.array_equal(run1[1], run0[1])
+        assert np.allclose(run1[2], run0[2], rtol=1e-12, atol=0)
+
+
 def test_relabel_off_and_on_agree():
     """Degree relabelling is internal: ids at the ABI are the caller's."""
     g = bcmi.Graph.rmat(11, 16, seed=21)
