@@ -222,6 +222,11 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   if (!key || !v) return set_error(BCMI_ERR_ARG, "null argument");
   std::string k(key);
   const Params& p = c->prm;
+  if (k == "label_passes") {  // read-only: union-find passes the uploaded graph's labels took
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    *v = c->g.cc_passes;
+    return BCMI_OK;
+  }
   if (k == "block") *v = p.block;
   else if (k == "grid") *v = p.grid;
   else if (k == "grid_plain") *v = p.grid_plain;

@@ -158,6 +158,9 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * graphs; 0: it launches the same kernels one by one, for profilers and
  * other tools that do not follow graph launches). */
 int bcmi_set_param(bcmi_ctx *ctx, const char *key, int64_t value);
+/* Reads a parameter above, or the read-only "label_passes": union and
+ * labelling passes the uploaded graph's component labels took (1 when the
+ * first pass already agrees with every arc; BCMI_ERR_STATE with no graph). */
 int bcmi_get_param(bcmi_ctx *ctx, const char *key, int64_t *value);
 
 /* Uploads a CSR graph (host pointers) and derives degrees and component
