@@ -113,7 +113,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20250101)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="back", choices=["back", "sigma", "td", "bu", "none"],
-                    help="kernel timed on the device for the roofline (none: no device timer)")
+                    help="kernel timed on the device: back (the roofline kernel), sigma, td or bu "
+                         "(time only), none (no device timer)")
     ap.add_argument("--graph", default="rmat", choices=["rmat", "chain"],
                     help="chain: bridged-block chain of --chain-blocks Erdos-Renyi blocks of 256 vertices, degree 8")
     ap.add_argument("--chain-blocks", type=int, default=32768)
@@ -233,23 +234,22 @@ def main():
     # Dominant-kernel roofline.  The batched byte model per batch is
     # B(S) = 24 m_S + 32 n_S + sum_s (24 a_s + 32 n_s); each of the two passes
     # (forward, backward) reads the CSR and probes masks once, so each gets half
-    # of the union part; the backward pass gets sum_s (16 a_s + 16 n_s)
-    # (coef of successors + sigma/coef of the vertex itself), the sigma pass
-    # sum_s (8 a_s + 16 n_s).
+    # of the union part; the backward pass (k_ms_back) gets sum_s (16 a_s + 16 n_s)
+    # (coef of successors + sigma/coef of the vertex itself).  The forward
+    # pass's share is spread over its kernels as the levels go (bottom-up levels
+    # sum path counts in their own scan, top-down levels in k_ms_sigma or
+    # k_ms_push), so a forward kernel gets its time but no byte figure.
     n_s, a_s = ss["n_s"].astype(np.float64), ss["a_s"].astype(np.float64)
     per_src = float(np.sum(24.0 * a_s + 32.0 * n_s))
     union_part = st["bytes"] - per_src
-    if args.kernel == "back":
-        kbytes = union_part / 2.0 + float(np.sum(16.0 * a_s + 16.0 * n_s))
-    elif args.kernel == "sigma":
-        kbytes = union_part / 2.0 + float(np.sum(8.0 * a_s + 16.0 * n_s))
-    else:
-        kbytes = union_part / 2.0
-    achieved = kbytes / (k_ms / 1e3) / 1e9 if k_ms > 0 and kt else None
+    kbytes = union_part / 2.0 + float(np.sum(16.0 * a_s + 16.0 * n_s)) if args.kernel == "back" else None
+    achieved = kbytes / (k_ms / 1e3) / 1e9 if kbytes and k_ms > 0 and kt else None
     # with pushed backward levels the backward timer also covers k_ms_bpush and k_ms_bfin
     kname = f"k_ms_{args.kernel}"
     if args.kernel == "back" and e.get_param("gamma_back") > 0:
         kname = "k_ms_back+k_ms_bpush+k_ms_bfin"
+    if args.kernel == "bu" and e.get_param("pack_bu"):
+        kname = "k_ms_bu+k_ms_bu_pk"  # the bottom-up timer covers both kernels
     log(f"kernel {args.kernel}: {k_launch} launches, {k_ms:.1f} ms total, "
         f"avg {k_ms / max(k_launch, 1) * 1e3:.1f} us, achieved {achieved} GB/s")
 
