@@ -176,10 +176,10 @@ def test_batch_width_and_many_batches(engine):
 @pytest.mark.parametrize("batching", [0, 1, 2, 3, 4, 5])
 def test_batch_orders_and_tuned_shape(batching):
     """Every source order (as given, hub depths, highest and second
-    highest-degree neighbours) with the launch shape tuned for R-MAT scale 22
-    (forward grid 2048, switch thresholds 1/8 and 1/128, pushes below 1/4 of
-    the arcs): BC within 1e-9 of the oracle; the order changes only how the
-    sources are grouped."""
+    highest-degree neighbours) with a large-graph launch shape (forward grid
+    2048, backward grid 16384, records of 512 arcs, switch thresholds 1/8 and
+    1/128, pushes below 1/4 of the arcs): BC within 1e-9 of the oracle; the
+    order changes only how the sources are grouped."""
     g = bcmi.Graph.rmat(12, 16, seed=31)
     src = g.sample_sources(700, seed=5)
     ref = O.brandes(g.off, g.adj, src, nthreads=8)
