@@ -5,7 +5,7 @@ space: record size (edges per work item, which needs a fresh upload), block
 shape, forward and backward grids, the top-down -> bottom-up switch
 threshold and the bottom-up floor, the frontier size below which path counts
 are pushed, bottom-up fusion and packing, pushed backward levels, the levels
-per captured graph and the batch width; then the plain sweep's switch
+per captured graph, the batch width and the path-count row width; then the plain sweep's switch
 threshold and grid.  (The small-grid size for sparse levels is left at the
 engine default, which sweeps on bridged-block chains set; R-MAT graphs do not
 depend on it.)
@@ -48,15 +48,16 @@ SPACE = {
     "gamma_back": [0, 4, 8, 16],
     "steps": [6, 8, 10, 12, 16],
     "width": [32, 48, 64],
+    "narrow_sigma": [1, 2],
 }
 PLAIN_SPACE = {"alpha_plain": [100, 300, 1000, 3000, 10000, 1 << 60], "grid_plain": [1024, 2048, 4096, 8192]}
 DEFAULTS = {"chunk": 256, "block": 256, "grid": 4096, "grid_back": 0, "alpha_batched": 1000, "beta_batched": 16,
             "gamma_push": 8, "fuse_bu": 2, "pack_bu": 1, "gamma_back": 0, "steps": 16, "width": 64,
-            "alpha_plain": 1000, "grid_plain": 4096}
+            "narrow_sigma": 1, "alpha_plain": 1000, "grid_plain": 4096}
 DEFAULT_PATH = os.path.join(HERE, "tuned.json")
 # engine parameters a table entry may set (the rest of an entry are notes)
 APPLIED = ("chunk", "block", "grid", "grid_back", "grid_plain", "alpha_batched", "beta_batched", "gamma_push",
-           "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width")
+           "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width", "narrow_sigma")
 NOTES = ("batched_gteps", "plain_gteps", "tuned_on", "tuned_by", "command", "sample")
 
 

@@ -89,7 +89,8 @@ static int param_set(Params& p, const std::string& k, int64_t v, bool have_graph
   } else if (k == "pack_bu") {
     p.pack_bu = v ? 1 : 0;
   } else if (k == "narrow_sigma") {
-    p.narrow_sigma = v ? 1 : 0;
+    if (v < 0 || v > 2) return set_error(BCMI_ERR_ARG, "narrow_sigma must be 0 (64-bit rows), 1 (32-bit first) or 2 (16-bit first)");
+    p.narrow_sigma = (int)v;
   } else if (k == "gamma_back") {
     if (v < 0) return set_error(BCMI_ERR_ARG, "gamma_back must be >= 0");
     p.gamma_back = v;

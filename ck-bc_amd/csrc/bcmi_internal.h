@@ -36,8 +36,9 @@ struct Params {
   int64_t gamma_push = 8;       // batched top-down levels with frontier arcs < all arcs / gamma push
                                 // path counts from the frontier (0: always pull from parents)
   int pack_bu = 1;              // batched bottom-up packs vertices of degree <= 16 four to a wave
-  int narrow_sigma = 1;         // exact batched sweeps first keep path counts in 32 bits (redone
-                                // with 64 bits when a count passes 2^32-1)
+  int narrow_sigma = 1;         // exact batched sweeps first keep path counts in 32-bit (1) or
+                                // 16-bit (2) rows, redone with the next wider rows when a count
+                                // passes the rows' range (0: 64-bit rows only)
   int64_t gamma_back = 0;       // batched backward levels whose successor level holds fewer than
                                 // 1/gamma_back of their arcs push coef from it (0: always pull)
   int steps = 16;           // levels per captured graph

@@ -102,7 +102,9 @@ struct DevGraph {
   // set once a batched sweep's exact path counts wrapped (auto mode then
   // starts later batches in scaled FP64)
   mutable std::atomic<bool> sigma_overflowed{false};
-  mutable std::atomic<bool> narrow_overflowed{false};  // a batch's counts passed 2^32-1
+  // narrowest path-count row width later batches start with (2 16-bit, 1
+  // 32-bit, 0 64-bit), lowered when a batch's counts pass a row's range
+  mutable std::atomic<int> narrow_cap{2};
   size_t bytes = 0;
   void release();
 };

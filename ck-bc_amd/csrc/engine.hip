@@ -313,17 +313,29 @@ struct BA {
   u64* lvmax;          // [64] running max of the level being built (double bits)
 };
 
-// Path-count rows.  Exact sweeps first keep the counts of a batch in 32-bit
-// rows ([v][64] of uint32, 256 bytes) - half the bytes of every gather; a
-// batch whose counts pass 2^32-1 is redone with 64-bit rows (exact integers
-// or FP64 bits when scaled).  The kernels are built for both widths (`nar` is
-// a template constant in their loops).
-__device__ __forceinline__ u64 sg_ld(const u64* sg, bool nar, int64_t i) {
-  return nar ? (u64)reinterpret_cast<const uint32_t*>(sg)[i] : sg[i];
+// Path-count rows.  Exact sweeps first keep the counts of a batch in narrow
+// rows: 16-bit ([v][64] of uint16, 128 bytes, one cache line per gather) or
+// 32-bit (256 bytes), as narrow_sigma asks; a batch whose counts pass the
+// row's range is redone with the next wider rows, up to 64-bit rows (exact
+// integers, or FP64 bits when scaled).  Width codes: 0 64-bit, 1 32-bit, 2
+// 16-bit; the kernels are built for each (`NW` is a template constant in
+// their loops).
+template <int NW> struct RowT { using T = u64; };
+template <> struct RowT<1> { using T = uint32_t; };
+template <> struct RowT<2> { using T = uint16_t; };
+template <int NW>
+__device__ __forceinline__ u64 sg_ld(const u64* sg, int64_t i) {
+  return (u64)reinterpret_cast<const typename RowT<NW>::T*>(sg)[i];
 }
-__device__ __forceinline__ void sg_st(u64* sg, bool nar, int64_t i, u64 x) {
-  if (nar) reinterpret_cast<uint32_t*>(sg)[i] = (uint32_t)x;
-  else sg[i] = x;
+template <int NW>
+__device__ __forceinline__ void sg_st(u64* sg, int64_t i, u64 x) {
+  reinterpret_cast<typename RowT<NW>::T*>(sg)[i] = (typename RowT<NW>::T)x;
+}
+// the same with the width known at run time
+__device__ __forceinline__ void sg_st_w(u64* sg, int nw, int64_t i, u64 x) {
+  if (nw == 2) sg_st<2>(sg, i, x);
+  else if (nw == 1) sg_st<1>(sg, i, x);
+  else sg_st<0>(sg, i, x);
 }
 // True on the lanes whose bit is set in the wave-uniform mask m: the mask
 // becomes the exec mask as it is, with no per-lane shift and compare.
@@ -332,37 +344,49 @@ __device__ __forceinline__ bool in_mask(u64 m) { return __builtin_amdgcn_inverse
 // their element of row `row` (exec = m), the others get 0.  The value keeps
 // the row's width, so the load writes its destination register directly and
 // the loads of a group stay in flight together until the first use.
-template <bool NAR> struct RowT { using T = u64; };
-template <> struct RowT<true> { using T = uint32_t; };
-template <bool NAR>
-__device__ __forceinline__ typename RowT<NAR>::T sg_gather(const u64* sg, u64 m, int32_t row) {
-  using T = typename RowT<NAR>::T;
+template <int NW>
+__device__ __forceinline__ typename RowT<NW>::T sg_gather(const u64* sg, u64 m, int32_t row) {
+  using T = typename RowT<NW>::T;
   T x = 0;
   if (in_mask(m)) x = reinterpret_cast<const T*>(sg)[(int64_t)row * 64 + lane_id()];
   return x;
 }
-// Adds a 32-bit term to a 64-bit count kept as two words (lo, hi).  The term
-// stays 32 bits wide up to the add, so the gather before it loads straight
-// into its register (a widened value would have to wait for the load inside
-// the gather's branch).
+// Adds a narrow term to a 64-bit count kept as two 32-bit words (lo, hi).
+// The term stays narrow up to the add, so the gather before it loads straight
+// into its register (a value widened to 64 bits would have to wait for the
+// load inside the gather's branch).
 __device__ __forceinline__ void add_w(uint32_t& lo, uint32_t& hi, uint32_t x) {
   uint32_t t;
   hi += __builtin_add_overflow(lo, x, &t);
   lo = t;
 }
-// stores an exact count; true when it does not fit the row
-__device__ __forceinline__ bool sg_put(u64* sg, bool nar, int64_t i, u64 x) {
-  sg_st(sg, nar, i, x);
-  return nar && x > 0xffffffffull;
+template <int NW>
+__device__ __forceinline__ constexpr u64 row_max() {
+  return NW == 2 ? 0xffffull : NW == 1 ? 0xffffffffull : ~0ull;
 }
-// adds an exact count atomically; true when the row's width overflows
-__device__ __forceinline__ bool sg_add(u64* sg, bool nar, int64_t i, u64 x) {
-  if (nar) {
+// stores an exact count; true when it does not fit the row
+template <int NW>
+__device__ __forceinline__ bool sg_put(u64* sg, int64_t i, u64 x) {
+  sg_st<NW>(sg, i, x);
+  return NW > 0 && x > row_max<NW>();
+}
+// Adds an exact count atomically; true when the row's width overflows.  A
+// 16-bit element is added through its 32-bit word (element 2k is the low
+// half): a carry out of it lands in the neighbouring element, which does not
+// matter, as the batch is then redone with wider rows.
+template <int NW>
+__device__ __forceinline__ bool sg_add(u64* sg, int64_t i, u64 x) {
+  if constexpr (NW == 2) {
+    const int sh = (int)(i & 1) * 16;
+    const uint32_t o = atomicAdd(reinterpret_cast<uint32_t*>(sg) + (i >> 1), (uint32_t)(x & 0xffffu) << sh);
+    return x > 0xffffull || ((o >> sh) & 0xffffu) + (uint32_t)x > 0xffffu;
+  } else if constexpr (NW == 1) {
     const uint32_t o = atomicAdd(reinterpret_cast<uint32_t*>(sg) + i, (uint32_t)x);
     return x > 0xffffffffull || o + (uint32_t)x < o;
+  } else {
+    const u64 o = atomicAdd(sg + i, x);
+    return o + x < o;
   }
-  const u64 o = atomicAdd(sg + i, x);
-  return o + x < o;
 }
 
 // Work-list appends are staged per wave in LDS and flushed with one global
@@ -917,7 +941,7 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled, int narrow) {
     c->arcs_seen = 0; c->sum_a = 0; c->n_td = 0; c->n_bu = 0;
     c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 0; c->uni_arcs = 0;
     c->sigma_ovf = 0; c->width = k; c->scaled = scaled; c->push = 0; c->bpush = 0;
-    c->narrow = narrow && !scaled;
+    c->narrow = scaled ? 0 : narrow;
     b.rec_lo[0] = 0;
   }
   b.st_n[i] = 0; b.st_m[i] = 0; b.st_a[i] = 0;
@@ -931,7 +955,7 @@ __global__ void k_ms_init(GA g, BA b, int k, int scaled, int narrow) {
     if (g.flags & KF_GATE) lb_set(b.lb, s);
     atomicOr(&b.seen[s], 1ull << i);
     atomicOr(&b.compmask[g.label[s]], 1ull << i);
-    sg_st(b.sigma, narrow && !scaled, (int64_t)s * 64 + i, scaled ? (u64)__double_as_longlong(1.0) : 1ull);
+    sg_st_w(b.sigma, scaled ? 0 : narrow, (int64_t)s * 64 + i, scaled ? (u64)__double_as_longlong(1.0) : 1ull);
   }
   __syncthreads();
   if (i < k) {
@@ -1077,7 +1101,7 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 #define PACK_DEG 16  // packed bottom-up: vertices of degree <= PACK_DEG ...
 #define PACK_W 4     // ... PACK_W to a wave (PACK_DEG * PACK_W = 64)
 static_assert(PACK_DEG * PACK_W == 64 && PACK_W == 4, "the packed bottom-up scan picks one vertex per sub-group of 16 lanes");
-template <bool NAR, bool PACKED>
+template <int NW, bool PACKED>
 __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone || c->mode != 1) return;
@@ -1091,7 +1115,6 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   const bool fuse = b.fuse;
   const bool scaled = c->scaled;
-  constexpr bool nar = NAR;
   const double inv_m = (fuse && scaled) ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   u64 ca = 0;
   int ovf = 0;
@@ -1149,20 +1172,20 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         if (!fr) continue;
         const u64 need = F & ~sv;  // u is a parent of v for these sources
         for_lanes<UNR_BU>(__ballot(need != 0), [&](const int (&q)[UNR_BU], const bool (&okq)[UNR_BU]) {
-          typename RowT<NAR>::T x[UNR_BU];
+          typename RowT<NW>::T x[UNR_BU];
 #pragma unroll
           for (int t = 0; t < UNR_BU; t++) {
             const int32_t uq = rl32(u, q[t]);
             const u64 nq = okq[t] ? rl64(need, q[t]) : 0ull;
-            x[t] = sg_gather<NAR>(b.sigma, nq, uq);
+            x[t] = sg_gather<NW>(b.sigma, nq, uq);
             cnt += in_mask(nq);
           }
           // the lanes outside a gather's mask add 0 (exact in both forms;
           // scaled counts always use 64-bit rows)
-          if (scaled && !NAR) {
+          if (scaled && NW == 0) {
 #pragma unroll
             for (int t = 0; t < UNR_BU; t++) facc = facc + __longlong_as_double((long long)x[t]);
-          } else if constexpr (NAR) {
+          } else if constexpr (NW > 0) {
             // 32-bit terms: a 64-bit sum of fewer than 2^32 of them cannot wrap
 #pragma unroll
             for (int t = 0; t < UNR_BU; t++) add_w(slo, shi, x[t]);
@@ -1177,7 +1200,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
         });
       }
       ca += cnt;
-      if (NAR) sacc = ((u64)shi << 32) | slo;
+      if (NW > 0) sacc = ((u64)shi << 32) | slo;
       acc = wave_or64(acc) & ~sv;
       bool won = false;
       int64_t dg = 0;
@@ -1210,9 +1233,9 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
             mx = fmax(mx, o + raw);
           }
         } else if (!split) {
-          if (sg_put(b.sigma, nar, (int64_t)v * 64 + lane, sacc)) ovf = 1;
+          if (sg_put<NW>(b.sigma, (int64_t)v * 64 + lane, sacc)) ovf = 1;
         } else {
-          if (sg_add(b.sigma, nar, (int64_t)v * 64 + lane, sacc)) ovf = 1;
+          if (sg_add<NW>(b.sigma, (int64_t)v * 64 + lane, sacc)) ovf = 1;
         }
       }
       app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
@@ -1282,12 +1305,12 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
               qa[t] = okq[t] ? __ffsll((long long)P) - 1 : 0;
               if (okq[t]) P &= P - 1;
             }
-            typename RowT<NAR>::T x[UNR_PK];
+            typename RowT<NW>::T x[UNR_PK];
 #pragma unroll
             for (int t = 0; t < UNR_PK; t++) {
               const int32_t uq = rl32(u, qa[t]);
               const u64 nq = okq[t] ? rl64(need, qa[t]) : 0ull;
-              x[t] = sg_gather<NAR>(b.sigma, nq, uq);
+              x[t] = sg_gather<NW>(b.sigma, nq, uq);
               ca += in_mask(nq);
             }
 #pragma unroll
@@ -1296,7 +1319,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
 #pragma unroll
               for (int q = 0; q < PACK_W; q++) {
                 if (gq != q || !okq[t]) continue;
-                if constexpr (NAR) {
+                if constexpr (NW > 0) {
                   add_w(wl[q], wh[q], x[t]);  // 32-bit terms cannot wrap a 64-bit sum
                 } else if (scaled) {
                   sq[q] = (u64)__double_as_longlong(__longlong_as_double((long long)sq[q]) +
@@ -1310,7 +1333,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
             }
           }
         }
-        if (NAR) {
+        if (NW > 0) {
 #pragma unroll
           for (int q = 0; q < PACK_W; q++) sq[q] = ((u64)wh[q] << 32) | wl[q];
         }
@@ -1333,7 +1356,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
               const double raw = __longlong_as_double((long long)sq[q]) * inv_m;
               b.sigma[row] = (u64)__double_as_longlong(raw);
               mx = fmax(mx, raw);
-            } else if (sg_put(b.sigma, nar, row, sq[q])) {
+            } else if (sg_put<NW>(b.sigma, row, sq[q])) {
               ovf = 1;
             }
           }
@@ -1352,8 +1375,10 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ms_bu(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  if (b.ctl->narrow) k_ms_bu_body<true, false>(g, b);
-  else k_ms_bu_body<false, false>(g, b);
+  const int nw = b.ctl->narrow;
+  if (nw == 2) k_ms_bu_body<2, false>(g, b);
+  else if (nw == 1) k_ms_bu_body<1, false>(g, b);
+  else k_ms_bu_body<0, false>(g, b);
   kt_exit(kt);
 }
 
@@ -1361,8 +1386,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 // 5 waves per SIMD, 96 VGPRs; its load chains need the extra resident wave).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ms_bu_pk(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  if (b.ctl->narrow) k_ms_bu_body<true, true>(g, b);
-  else k_ms_bu_body<false, true>(g, b);
+  const int nw = b.ctl->narrow;
+  if (nw == 2) k_ms_bu_body<2, true>(g, b);
+  else if (nw == 1) k_ms_bu_body<1, true>(g, b);
+  else k_ms_bu_body<0, true>(g, b);
   kt_exit(kt);
 }
 
@@ -1370,7 +1397,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 // neighbours u that are at depth d for source i (the parents).  The first
 // record of v publishes its mask, marks it seen and counts statistics.  Also
 // clears level d-1 from its rotating array (not read at this step).
-template <bool NAR>
+template <int NW>
 __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone) return;
@@ -1389,7 +1416,6 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
   const bool scaled = c->scaled;
   const bool push = c->push;  // k_ms_push sums the parents
   const bool fused = c->mode == 1 && b.fuse;  // k_ms_bu summed the parents (of unsplit vertices if fuse 1)
-  constexpr bool nar = NAR;
   const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   double mx = 0.0;
   // After a bottom-up level that summed every vertex's parents, or before a
@@ -1430,7 +1456,7 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
           cm += dq;
           const int64_t row = (int64_t)rl32(v, q) * 64 + lane;
           if (dq > g.chunk) b.coef[row] = 0.0;
-          if (push) sg_st(b.sigma, nar, row, 0);  // k_ms_push adds the parents' counts next
+          if (push) sg_st<NW>(b.sigma, row, 0);  // k_ms_push adds the parents' counts next
         }
       }
     }
@@ -1471,18 +1497,18 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
       // Parents UNR at a time: the row loads are issued before any is
       // consumed, and the sums still follow adjacency order.
       for_lanes<UNR>(__ballot(F != 0), [&](const int (&q)[UNR], const bool (&ok)[UNR]) {
-        typename RowT<NAR>::T x[UNR];
+        typename RowT<NW>::T x[UNR];
 #pragma unroll
         for (int t = 0; t < UNR; t++) {
           const int32_t uq = rl32(u, q[t]);
           const u64 fq = ok[t] ? rl64(F, q[t]) : 0ull;
-          x[t] = sg_gather<NAR>(b.sigma, fq, uq);
+          x[t] = sg_gather<NW>(b.sigma, fq, uq);
           ca += in_mask(fq);
         }
-        if (scaled && !NAR) {
+        if (scaled && NW == 0) {
 #pragma unroll
           for (int t = 0; t < UNR; t++) accf = accf + __longlong_as_double((long long)x[t]);
-        } else if constexpr (NAR) {
+        } else if constexpr (NW > 0) {
 #pragma unroll
           for (int t = 0; t < UNR; t++) add_w(alo, ahi, x[t]);
         } else {
@@ -1495,12 +1521,12 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
         }
       });
     }
-    if (NAR) acc = ((u64)ahi << 32) | alo;
+    if (NW > 0) acc = ((u64)ahi << 32) | alo;
     u64* row = b.sigma + (int64_t)v * 64 + lane;
     if (done_by_bu) {
       // row written by k_ms_bu
     } else if (push) {
-      if (mine && j == 0) sg_st(b.sigma, nar, (int64_t)v * 64 + lane, 0);  // k_ms_push adds the parents' counts next
+      if (mine && j == 0) sg_st<NW>(b.sigma, (int64_t)v * 64 + lane, 0);  // k_ms_push adds the parents' counts next
     } else if (scaled) {
       double raw = accf * inv_m;
       if (mine) {
@@ -1509,9 +1535,9 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
         mx = fmax(mx, raw);
       }
     } else if (nch == 1) {
-      if (mine && sg_put(b.sigma, nar, (int64_t)v * 64 + lane, acc)) ovf = 1;
+      if (mine && sg_put<NW>(b.sigma, (int64_t)v * 64 + lane, acc)) ovf = 1;
     } else if (mine) {
-      if (sg_add(b.sigma, nar, (int64_t)v * 64 + lane, acc)) ovf = 1;
+      if (sg_add<NW>(b.sigma, (int64_t)v * 64 + lane, acc)) ovf = 1;
     }
     if (j == 0) {
       u64 so = b.seen[v];
@@ -1551,8 +1577,10 @@ __device__ __forceinline__ void k_ms_sigma_body(GA g, BA b) {
 
 __global__ void __launch_bounds__(256) k_ms_sigma(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  if (b.ctl->narrow) k_ms_sigma_body<true>(g, b);
-  else k_ms_sigma_body<false>(g, b);
+  const int nw = b.ctl->narrow;
+  if (nw == 2) k_ms_sigma_body<2>(g, b);
+  else if (nw == 1) k_ms_sigma_body<1>(g, b);
+  else k_ms_sigma_body<0>(g, b);
   kt_exit(kt);
 }
 
@@ -1563,7 +1591,7 @@ __global__ void __launch_bounds__(256) k_ms_sigma(GA g, BA b, KT* kt) {
 // level's, which the pull in k_ms_sigma would scan.  Exact counts use
 // returning atomics to detect a wrap; scaled counts add raw values and keep
 // the level maximum (every returned sum is a lower bound of a final one).
-template <bool NAR>
+template <int NW>
 __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (c->fdone || !c->push) return;
@@ -1574,7 +1602,6 @@ __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   const bool scaled = c->scaled;
-  constexpr bool nar = NAR;
   const double inv_m = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   u64 ca = 0;
   int ovf = 0;
@@ -1585,7 +1612,7 @@ __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
     const u64 F = b.rec_mask[r - j];
     const int64_t e0 = g.off[u] + (int64_t)j * g.chunk;
     const int64_t e1 = min(g.off[u + 1], e0 + g.chunk);
-    const u64 sb = ((F >> lane) & 1) ? sg_ld(b.sigma, nar, (int64_t)u * 64 + lane) : 0ull;
+    const u64 sb = ((F >> lane) & 1) ? sg_ld<NW>(b.sigma, (int64_t)u * 64 + lane) : 0ull;
     const double sf = scaled ? __longlong_as_double((long long)sb) * inv_m : 0.0;
     for (int64_t eb = e0; eb < e1; eb += 64) {
       const int64_t e = eb + lane;
@@ -1607,7 +1634,7 @@ __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
           if (scaled) {
             double o = atomicAdd((double*)row, sf);
             mx = fmax(mx, o + sf);
-          } else if (sg_add(b.sigma, nar, (int64_t)vq * 64 + lane, sb)) {
+          } else if (sg_add<NW>(b.sigma, (int64_t)vq * 64 + lane, sb)) {
             ovf = 1;
           }
         }
@@ -1620,8 +1647,10 @@ __device__ __forceinline__ void k_ms_push_body(GA g, BA b) {
 }
 
 __global__ void __launch_bounds__(256) k_ms_push(GA g, BA b) {
-  if (b.ctl->narrow) k_ms_push_body<true>(g, b);
-  else k_ms_push_body<false>(g, b);
+  const int nw = b.ctl->narrow;
+  if (nw == 2) k_ms_push_body<2>(g, b);
+  else if (nw == 1) k_ms_push_body<1>(g, b);
+  else k_ms_push_body<0>(g, b);
 }
 
 // Backward level d for a batch: lane i accumulates, in adjacency order,
@@ -1631,7 +1660,7 @@ __global__ void __launch_bounds__(256) k_ms_push(GA g, BA b) {
 // publishes level d into lm[d%3] for the next (shallower) step and clears
 // level d+2 from lm[(d+2)%3].
 // Backward level d, one record (the pull): see k_ms_back_body.
-template <bool NAR>
+template <int NW>
 __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, int j, u64 L, int64_t o0, int64_t o1,
                                          int d, const u64* lmn, u64* lmd, const unsigned* nlb, unsigned* dlb,
                                          double mult, bool scaled, double inv_md) {
@@ -1642,7 +1671,7 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
   int64_t e1 = min(o1, e0 + g.chunk);
   const bool mine = (L >> lane) & 1;
   const double wv = g.w ? (double)g.w[v] : 0.0;
-  const u64 sbits = mine ? sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane) : 1ull;
+  const u64 sbits = mine ? sg_ld<NW>(b.sigma, (int64_t)v * 64 + lane) : 1ull;
   const double svr = !mine ? 1.0 : scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
   const double sv = scaled ? svr * inv_md : svr;
   double acc = (nch == 1) ? wv : 0.0;
@@ -1690,7 +1719,7 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
   }
 }
 
-template <bool NAR>
+template <int NW>
 __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (!c->fdone || c->bdone) return;
@@ -1739,7 +1768,7 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
       while (R) {
         const int q = __ffsll((long long)R) - 1;
         R &= R - 1;
-        back_one<NAR>(g, b, rl32(vq, q), rl32(jq, q), rl64(Lq, q), (int64_t)rl64((u64)o0q, q),
+        back_one<NW>(g, b, rl32(vq, q), rl32(jq, q), rl64(Lq, q), (int64_t)rl64((u64)o0q, q),
                       (int64_t)rl64((u64)o1q, q), d, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
       }
     }
@@ -1765,8 +1794,10 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BACK_WAVES))) k_ms_back(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  if (b.ctl->narrow) k_ms_back_body<true>(g, b);
-  else k_ms_back_body<false>(g, b);
+  const int nw = b.ctl->narrow;
+  if (nw == 2) k_ms_back_body<2>(g, b);
+  else if (nw == 1) k_ms_back_body<1>(g, b);
+  else k_ms_back_body<0>(g, b);
   kt_exit(kt);
 }
 
@@ -1823,7 +1854,7 @@ __global__ void __launch_bounds__(256) k_ms_bpush(GA g, BA b, KT* kt) {
 // pushed level, every level-d vertex (delta = w(v) + sigma(v) x the pushed
 // sum of its successors' coef).
 // One such vertex (wave-wide; lanes = sources).
-template <bool NAR>
+template <int NW>
 __device__ __forceinline__ void bfin_one(const GA& g, const BA& b, int64_t r, int32_t v, bool bpush, bool scaled,
                                          double inv_md, double mult) {
   const int lane = lane_id();
@@ -1832,7 +1863,7 @@ __device__ __forceinline__ void bfin_one(const GA& g, const BA& b, int64_t r, in
   double contrib = 0.0;
   if (mine) {
     double* crow = b.coef + (int64_t)v * 64 + lane;
-    u64 sbits = sg_ld(b.sigma, NAR, (int64_t)v * 64 + lane);
+    u64 sbits = sg_ld<NW>(b.sigma, (int64_t)v * 64 + lane);
     double svr = scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
     double part = *crow;
     if (bpush) part = part * (scaled ? svr * inv_md : svr);
@@ -1845,7 +1876,7 @@ __device__ __forceinline__ void bfin_one(const GA& g, const BA& b, int64_t r, in
   if (lane == 0) b.bc[v] += contrib;
 }
 
-template <bool NAR>
+template <int NW>
 __device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
   Ctl* c = b.ctl;
   if (!c->fdone || c->bdone) return;
@@ -1860,7 +1891,7 @@ __device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
   const double inv_md = scaled ? 1.0 / b.lscale[(int64_t)d * 64 + lane] : 1.0;
   if (bpush) {
     for (int64_t r = lo + gw; r < hi; r += tw)
-      if (b.rec_j[r] == 0) bfin_one<NAR>(g, b, r, b.rec_v[r], true, scaled, inv_md, mult);
+      if (b.rec_j[r] == 0) bfin_one<NW>(g, b, r, b.rec_v[r], true, scaled, inv_md, mult);
     return;
   }
   // only split vertices are left: find their first records 64 at a time
@@ -1876,15 +1907,17 @@ __device__ __forceinline__ void k_ms_bfin_body(GA g, BA b) {
     while (S) {
       const int q = __ffsll((long long)S) - 1;
       S &= S - 1;
-      bfin_one<NAR>(g, b, base + q, rl32(v, q), false, scaled, inv_md, mult);
+      bfin_one<NW>(g, b, base + q, rl32(v, q), false, scaled, inv_md, mult);
     }
   }
 }
 
 __global__ void __launch_bounds__(256) k_ms_bfin(GA g, BA b, KT* kt) {
   kt_enter(kt);
-  if (b.ctl->narrow) k_ms_bfin_body<true>(g, b);
-  else k_ms_bfin_body<false>(g, b);
+  const int nw = b.ctl->narrow;
+  if (nw == 2) k_ms_bfin_body<2>(g, b);
+  else if (nw == 1) k_ms_bfin_body<1>(g, b);
+  else k_ms_bfin_body<0>(g, b);
   kt_exit(kt);
 }
 
@@ -1918,7 +1951,7 @@ __global__ void k_ms_cleanup(GA g, BA b) {
   // the path-count rows of split vertices are zero outside a sweep
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.nsplit * 64;
        t += (int64_t)gridDim.x * blockDim.x)
-    sg_st(b.sigma, c->narrow, (int64_t)g.split_v[t >> 6] * 64 + (t & 63), 0);
+    sg_st_w(b.sigma, c->narrow, (int64_t)g.split_v[t >> 6] * 64 + (t & 63), 0);
 }
 
 // ====================================== workgroup-per-source sweeps
@@ -2396,7 +2429,7 @@ void DevGraph::release() {
   fr(round); fr(parent); fr(w); fr(root); fr(order); fr(d_n2o);
   d_n2o = nullptr; relabeled = false; h_n2o.clear(); h_o2n.clear();
   sigma_overflowed.store(false);
-  narrow_overflowed.store(false);
+  narrow_cap.store(2);
   off = foff = toff = nullptr; adj = fadj = tadj = nullptr; label = nullptr; comp_size = comp_arcs = nullptr;
   all_v = all_j = split_v = sm_v = lg_v = lg_j = nullptr; nsplit = nsm = nlg = 0; round = parent = root = order = nullptr; w = nullptr;
   n = m = fm = nall = 0; nrounds = 0; round_lo.clear(); h_off.clear(); h_round.clear(); h_sig.clear(); bytes = 0;
@@ -2774,7 +2807,7 @@ struct Workspace {
   std::vector<Workspace*> helpers;
   const DevGraph* helpers_graph = nullptr;
   // workgroup-per-source sweeps (deep graphs): per-slot state and per-run buffers
-  int sig_layout = -1;  // layout of the last batched sweep's path-count rows: 1 narrow, 0 wide, -1 all zero
+  int sig_layout = -1;  // row width code of the last batched sweep's path counts (2, 1, 0), -1 all zero
   int64_t wg_slots = 0, wg_lcap = 0;
   int64_t wg_req = 0;                 // slots asked for (wg_slots may be fewer: device memory)
   int32_t* wg_depth = nullptr;
@@ -3183,7 +3216,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
     } else {
       // Split vertices' rows are zero between sweeps in the layout in use;
       // a change of layout starts from an all-zero array.
-      const int lay = narrow && !scaled ? 1 : 0;
+      const int lay = scaled ? 0 : narrow;
       if (ws->sig_layout >= 0 && ws->sig_layout != lay)
         hipLaunchKernelGGL(k_fill_u64, grid_for(64 * ws->n), 256, 0, st, ws->sigma, 0ull, 64 * ws->n);
       ws->sig_layout = lay;
@@ -3389,6 +3422,13 @@ static int run_plain_one(const DevGraph& g, Workspace* w, const Params& prm, int
 }
 
 // One batched sweep over k <= 64 sources (internal ids) with multipliers.
+// Later batches of g start with rows no narrower than width code nw.
+static void lower_narrow_cap(const DevGraph& g, int nw) {
+  int cap = g.narrow_cap.load();
+  while (cap > nw && !g.narrow_cap.compare_exchange_weak(cap, nw)) {
+  }
+}
+
 static int run_batch(const DevGraph& g, Workspace* w, const Params& prm, const int32_t* src, const double* mult,
                      int k, RunPart& P) {
   const double t0 = now_ms();
@@ -3402,13 +3442,14 @@ static int run_batch(const DevGraph& g, Workspace* w, const Params& prm, const i
   int lv = 0;
   // auto mode: once a batch of this graph has overflowed, later ones start scaled
   int scaled = prm.sigma == 1 || (prm.sigma == 2 && g.sigma_overflowed.load());
-  const int narrow = !scaled && prm.narrow_sigma && !g.narrow_overflowed.load();
+  int narrow = scaled ? 0 : std::min(prm.narrow_sigma, g.narrow_cap.load());
   TRY(sweep(g, w, prm, k, false, &lv, scaled, narrow));
-  if (narrow && w->hctl->sigma_ovf) {
-    // a count passed 2^32-1 (the backward pass was skipped): redo the batch
-    // with 64-bit rows, as later batches of this graph will start
-    g.narrow_overflowed.store(true);
-    TRY(sweep(g, w, prm, k, false, &lv, 0, 0));
+  while (narrow > 0 && w->hctl->sigma_ovf) {
+    // a count passed the row's range (the backward pass was skipped): redo the
+    // batch with the next wider rows, which later batches of this graph start with
+    narrow--;
+    lower_narrow_cap(g, narrow);
+    TRY(sweep(g, w, prm, k, false, &lv, 0, narrow));
   }
   if (w->hctl->sigma_ovf && prm.sigma == 2) {
     // exact counts wrapped: the backward pass was skipped, redo the batch scaled
@@ -4002,15 +4043,17 @@ int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32
   HIP_TRY(hipMemcpyAsync(ws->srcbuf, ws->hsrc, 64 * sizeof(int32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(ws->multbuf, ws->hmult, 64 * sizeof(double), hipMemcpyHostToDevice, st));
   int lv;
-  const int narrow = prm.narrow_sigma && !g.narrow_overflowed.load();
+  int narrow = std::min(prm.narrow_sigma, g.narrow_cap.load());
   int rc = sweep(g, ws, prm, k, true, &lv, 0, narrow);
-  if (rc == BCMI_OK && narrow && ws->hctl->sigma_ovf) {
-    // counts past 2^32-1: redo with 64-bit rows
+  while (rc == BCMI_OK && narrow > 0 && ws->hctl->sigma_ovf) {
+    // counts past the row's range: redo with the next wider rows
+    narrow--;
+    lower_narrow_cap(g, narrow);
     rc = sweep_cleanup(g, ws, prm);
-    if (rc == BCMI_OK) rc = sweep(g, ws, prm, k, true, &lv, 0, 0);
+    if (rc == BCMI_OK) rc = sweep(g, ws, prm, k, true, &lv, 0, narrow);
   }
   if (rc != BCMI_OK) return rc;
-  const bool nar = ws->hctl->narrow;
+  const int nw = ws->hctl->narrow;
   int64_t nrec = std::min<int64_t>(ws->hctl->rec_count, ws->cap);
   int last = ws->hctl->last;
   std::vector<int32_t> rv(nrec), rj(nrec);
@@ -4025,11 +4068,12 @@ int msbfs_debug(const DevGraph& g, Workspace* ws, const Params& prm, const int32
   HIP_TRY(hipMemcpyAsync(sg.data(), ws->sigma, sizeof(u64) * 64 * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(dd.data(), ws->dbg_delta, sizeof(double) * 64 * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  if (nar) {  // 32-bit rows: widen in place, from the end
-    const char* s32 = reinterpret_cast<const char*>(sg.data());
+  if (nw > 0) {  // narrow rows: widen in place, from the end
+    const char* sn = reinterpret_cast<const char*>(sg.data());
+    const size_t wb = nw == 2 ? 2 : 4;
     for (size_t i = (size_t)64 * n; i-- > 0;) {
-      uint32_t x;
-      memcpy(&x, s32 + 4 * i, 4);
+      uint32_t x = 0;
+      memcpy(&x, sn + wb * i, wb);
       sg[i] = x;
     }
   }

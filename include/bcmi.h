@@ -130,7 +130,8 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * degree <= 16 four to a wave in a kernel of their own; 0: one per wave),
  * "narrow_sigma" (1, the default: exact batched sweeps first keep path
  * counts in 32-bit rows and redo a batch with 64-bit rows when a count
- * passes 2^32-1; 0: 64-bit rows only),
+ * passes 2^32-1; 2: 16-bit rows first, then 32-bit, then 64-bit; 0: 64-bit
+ * rows only),
  * "gamma_back" (batched backward levels whose successor level holds
  * fewer than 1/gamma_back of their arcs take their sums pushed from the
  * successors; 0, the default, always pulls),

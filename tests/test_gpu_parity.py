@@ -369,16 +369,18 @@ def test_batched_level_variants(gamma_back, fuse_bu, chunk, pack_bu, sigma):
             e.close()
 
 
-@pytest.mark.parametrize("narrow", [1, 0])
-def test_batched_counts_past_32_bits(narrow):
-    """Hypercube Q13: sigma(s -> v) = popcount(s ^ v)!, and 13! > 2^32 - 1.
-    With 32-bit rows first (narrow_sigma 1) the batch is redone with 64-bit
-    rows: depths and path counts still match the closed form exactly, BC
-    matches the oracle, and a later run on the same graph agrees."""
+@pytest.mark.parametrize("d", [13, 9])
+@pytest.mark.parametrize("narrow", [2, 1, 0])
+def test_batched_counts_past_narrow_rows(narrow, d):
+    """Hypercubes Q13 and Q9: sigma(s -> v) = popcount(s ^ v)!, with 13! >
+    2^32 - 1 and 2^16 - 1 < 9! < 2^32.  With 16-bit (narrow_sigma 2) or
+    32-bit rows (1) first, a batch whose counts pass the rows' range is
+    redone with the next wider rows (Q13 from 16 bits: twice): depths and
+    path counts still match the closed form exactly, BC matches the oracle,
+    and a later run on the same graph agrees."""
     from math import factorial
-    d = 13
     g = bcmi.Graph.from_edges(*G.hypercube(d))
-    src = np.array([0, 5, 1234, 8191, 4096, 77], np.int32)
+    src = np.array([0, 5, 1234, 8191, 4096, 77], np.int32) % g.n
     e = bcmi.Engine(narrow_sigma=narrow)
     e.upload(g)
     dm, sgm, _ = e.msbfs(src)
@@ -389,6 +391,29 @@ def test_batched_counts_past_32_bits(narrow):
         assert np.array_equal(dm[q], pc)
         assert np.array_equal(sgm[q], fact[pc])
     ref = O.brandes(g.off, g.adj, src, nthreads=8)
+    for _ in range(2):
+        e.reset()
+        e.run(src, mode=bcmi.MODE_BATCHED)
+        assert bcmi.rel_err(e.bc(), ref) <= BC_TOL
+    e.close()
+
+
+@pytest.mark.parametrize("fuse_bu,pack_bu", [(2, 1), (1, 0)])
+def test_sixteen_bit_rows(fuse_bu, pack_bu):
+    """16-bit path-count rows first (narrow_sigma 2) on graphs whose counts fit
+    them: records of 64 arcs split the hubs (their partial sums are added
+    atomically through 32-bit words), top-down and bottom-up levels: depths and
+    path counts bit-exact, BC within 1e-9 of the oracle, twice in a row."""
+    g = bcmi.Graph.rmat(12, 16, seed=3)
+    src = g.sample_sources(200, seed=11)
+    ref = O.brandes(g.off, g.adj, src, nthreads=8)
+    e = bcmi.Engine(narrow_sigma=2, chunk=64, fuse_bu=fuse_bu, pack_bu=pack_bu)
+    e.upload(g)
+    d, sg, _ = e.msbfs(src[:64])
+    for q in range(0, 64, 7):
+        od, osg, _, _ = O.sssp(g.off, g.adj, int(src[q]))
+        assert int(osg.max()) < 1 << 16
+        assert np.array_equal(d[q], od) and np.array_equal(sg[q], osg), q
     for _ in range(2):
         e.reset()
         e.run(src, mode=bcmi.MODE_BATCHED)
