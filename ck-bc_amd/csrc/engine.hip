@@ -1094,6 +1094,12 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 // kernel skips its parent scan: a vertex held in one record stores its sum,
 // the records of a split vertex (fuse_bu 2) add theirs atomically into its
 // row, whose columns are zero until the source reaches it.
+#ifndef BU_PIPE
+#define BU_PIPE 1  // bottom-up slabs' adjacency and mask loads run ahead of the gathers
+#endif
+#ifndef BU_WAVES
+#define BU_WAVES 5
+#endif
 #ifndef UNR_BU
 #define UNR_BU 12  // parent gathers in flight per wave in the bottom-up kernel
 #endif
@@ -1160,7 +1166,27 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       double facc = 0.0;
       uint32_t slo = 0, shi = 0;  // 32-bit rows: the exact sum as two words
       uint32_t cnt = 0;  // this record's parents per source (at most its arcs)
+#if BU_PIPE
+      // The slabs' dependent loads run ahead of the sums: slab k+1's level
+      // masks and slab k+2's adjacency are in flight while slab k's parents
+      // are gathered.
+      int32_t u_c = 0, u_n = 0;
+      u64 F_c = 0;
+      if (e0 + lane < e1) u_c = g.adj[e0 + lane];
+      if (e0 + 64 + lane < e1) u_n = g.adj[e0 + 64 + lane];
+      if (e0 + lane < e1) F_c = cur[u_c];
+#endif
       for (int64_t eb = e0; eb < e1; eb += 64) {
+#if BU_PIPE
+        const int32_t u = u_c;
+        const u64 F = F_c;
+        F_c = 0;
+        if (eb + 64 + lane < e1) F_c = cur[u_n];
+        u_c = u_n;
+        u_n = 0;
+        if (eb + 128 + lane < e1) u_n = g.adj[eb + 128 + lane];
+        acc |= F;
+#else
         int64_t e = eb + lane;
         int32_t u = 0;
         u64 F = 0;
@@ -1169,6 +1195,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
           F = cur[u];
           acc |= F;
         }
+#endif
         if (!fr) continue;
         const u64 need = F & ~sv;  // u is a parent of v for these sources
         for_lanes<UNR_BU>(__ballot(need != 0), [&](const int (&q)[UNR_BU], const bool (&okq)[UNR_BU]) {
@@ -1373,7 +1400,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   }
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ms_bu(GA g, BA b, KT* kt) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BU_WAVES))) k_ms_bu(GA g, BA b, KT* kt) {
   kt_enter(kt);
   const int nw = b.ctl->narrow;
   if (nw == 2) k_ms_bu_body<2, false>(g, b);
