@@ -21,6 +21,7 @@ ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_OVERFLOW, ERR_STATE, ERR_IO, ERR_RANGE, ERR_BUS
 MODE_PLAIN, MODE_BATCHED = 0, 1
 H_NONE, H_PEEL = 0, 1
 SRC_EXACT, SRC_EXPAND = 0, 1
+SRC_DEG2 = 2  # OR-ed into the source flags: 2-degree heuristic (plain sweeps)
 
 EXPORTS = [
     "bcmi_last_error", "bcmi_version",
@@ -52,7 +53,8 @@ class RunStats(ctypes.Structure):
                 ("bytes", ctypes.c_double), ("gpu_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("max_levels", ctypes.c_int64), ("sum_n", ctypes.c_int64), ("sum_m", ctypes.c_int64),
                 ("sum_a", ctypes.c_int64), ("sum_pairs", ctypes.c_int64), ("sum_union", ctypes.c_int64),
-                ("scaled_batches", ctypes.c_int64), ("wg_sources", ctypes.c_int64)]
+                ("scaled_batches", ctypes.c_int64), ("wg_sources", ctypes.c_int64),
+                ("deg2_sources", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -280,11 +282,11 @@ class Engine:
     def reset(self):
         _check(lib().bcmi_bc_reset(self._c))
 
-    def run(self, sources, mode=MODE_BATCHED, expand=False):
+    def run(self, sources, mode=MODE_BATCHED, expand=False, deg2=False):
         src = np.ascontiguousarray(sources, dtype=np.int32)
         st = RunStats()
-        _check(lib().bcmi_bc_run(self._c, _p(src), src.shape[0], int(mode), SRC_EXPAND if expand else SRC_EXACT,
-                                 ctypes.byref(st)))
+        flags = (SRC_EXPAND if expand else SRC_EXACT) | (SRC_DEG2 if deg2 else 0)
+        _check(lib().bcmi_bc_run(self._c, _p(src), src.shape[0], int(mode), flags, ctypes.byref(st)))
         return st.as_dict()
 
     def run_static(self, sources):

@@ -124,6 +124,9 @@ static int param_set(Params& p, const std::string& k, int64_t v, bool have_graph
   } else if (k == "wg_slots") {
     if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "wg_slots must be in [1,65536]");
     p.wg_slots = (int)v;
+  } else if (k == "deg2_slots") {
+    if (v < 0 || v > 4096) return set_error(BCMI_ERR_ARG, "deg2_slots must be in [0,4096]");
+    p.deg2_slots = (int)v;
   } else if (k == "streams") {
     if (v < 1 || v > 16) return set_error(BCMI_ERR_ARG, "streams must be in [1,16]");
     p.streams = (int)v;
@@ -258,6 +261,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "wg_depth") *v = p.wg_depth;
   else if (k == "wg_mem") *v = p.wg_mem;
   else if (k == "wg_slots") *v = p.wg_slots;
+  else if (k == "deg2_slots") *v = p.deg2_slots;
   else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
   return BCMI_OK;
 }
@@ -339,6 +343,7 @@ int bcmi_bc_run(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int mode, int
   if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
   if (nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad source list");
   if (mode != BCMI_MODE_PLAIN && mode != BCMI_MODE_BATCHED) return set_error(BCMI_ERR_ARG, "bad mode");
+  if (src_flags & ~(BCMI_SRC_EXPAND | BCMI_SRC_DEG2)) return set_error(BCMI_ERR_ARG, "bad source flags");
   std::lock_guard<std::mutex> lk(c->mu);
   int rc = use_device(c);
   if (rc) return rc;
@@ -363,6 +368,7 @@ int bcmi_bc_run(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int mode, int
     st->sum_pairs = R.pairs;
     st->scaled_batches = R.scaled_batches;
     st->wg_sources = R.wg_sources;
+    st->deg2_sources = R.deg2_sources;
     for (int64_t u : R.uni_n) st->sum_union += u;
   }
   return rc;

@@ -38,7 +38,7 @@ static void die(const char* fmt, ...) {
 
 static void usage(const char* p) {
   die("Usage (BC-1GPU):\n"
-      "\t %s -p 1x1 [-o outfile] [-D] [-m] [-N <# of searches>] [-H 0|1]\n"
+      "\t %s -p 1x1 [-o outfile] [-D] [-m] [-N <# of searches>] [-H 0|1|2|3]\n"
       "\t -> to visit a graph read from file:\n"
       "\t\t -f <graph file> -n <# vertices> [-r <start vert>]\n"
       "\t -> to visit an RMAT graph:\n"
@@ -50,7 +50,9 @@ static void usage(const char* p) {
       "\t\t -s <seed>          sample -N random sources\n"
       "\t\t -F <file>          read the sources from a file\n"
       "\t\t -P                 full-precision result file\n"
-      "\t\t -X                 with -H 1: exact BC over exactly the listed sources\n"
+      "\t\t -X                 with -H 1|3: exact BC over exactly the listed sources\n"
+      "\t\t -H 2|3             2-degree sources taken from their neighbours' sweeps\n"
+      "\t\t                    (with -B plain; -H 3 also peels degree-1 vertices)\n"
       "\t\t -C <file>          binary CSR cache of the -f graph: used when it matches\n"
       "\t\t                    the text file's size and time, (re)written otherwise\n"
       "\t\t -T <key>=<value>   engine tunable (integer; repeatable), e.g. -T grid=2048\n"
@@ -155,7 +157,11 @@ int main(int argc, char** argv) {
   if (R * C != 1) die("This build runs on one GPU: use -p 1x1.\n");
   if (startv >= N) die("Invalid start vertex: %" PRIu64 ".\n", startv);
   if (heuristic < 0 || heuristic > 3) die("Invalid Heuristic Option (-H): %d\n", heuristic);
-  if (heuristic == 1 && !undirected) die("The 1-degree reduction (-H 1) needs an undirected graph.\n");
+  if ((heuristic & 1) && !undirected) die("The 1-degree reduction (-H 1, -H 3) needs an undirected graph.\n");
+  // -H 1 and -H 3 peel pendant vertices; -H 2 and -H 3 take degree-2 sources
+  // from their neighbours' sweeps (plain sweeps; bc2d.c:759-760 accepts both and
+  // prints "REMOVED" for them, c:1015)
+  const bool peel = heuristic == 1 || heuristic == 3, deg2 = heuristic >= 2;
 
   printf("\n\n****** MI355X (gfx950) BC ******\n\n");
   printf("Total number of vertices (N): %" PRIu64 "\n", N);
@@ -166,6 +172,8 @@ int main(int argc, char** argv) {
   printf("\n\n");
   if (heuristic == 0) printf("HEURISTICs: OFF: %d\n", heuristic);
   else if (heuristic == 1) printf("HEURISTICs: 1-degree reduction ON: %d\n", heuristic);
+  else if (heuristic == 2) printf("HEURISTICs: 2-degree reduction ON: %d\n", heuristic);
+  else printf("HEURISTICs: 1-degree and 2-degree reduction ON: %d\n", heuristic);
   printf("SWEEP: %s\n\n", mode == BCMI_MODE_PLAIN ? "plain (one source per sweep)" : "batched (64 sources per sweep)");
   std::string resname;
   if (!ofile.empty()) {
@@ -235,10 +243,10 @@ int main(int argc, char** argv) {
   CK(bcmi_create(0, &ctx));
   for (const auto& t : tunables) CK(bcmi_set_param(ctx, t.first.c_str(), (int64_t)t.second));
   t0 = now_s();
-  CK(bcmi_upload_graph(ctx, g->n, g->off, g->adj, g->directed, heuristic == 1 ? BCMI_H_PEEL : BCMI_H_NONE));
-  double redu_time = heuristic == 1 ? now_s() - t0 : 0.0;
+  CK(bcmi_upload_graph(ctx, g->n, g->off, g->adj, g->directed, peel ? BCMI_H_PEEL : BCMI_H_NONE));
+  double redu_time = peel ? now_s() - t0 : 0.0;
   std::vector<int32_t> round;
-  if (heuristic == 1) {
+  if (peel) {
     round.resize(g->n);
     int32_t nr = 0;
     CK(bcmi_get_peel(ctx, round.data(), nullptr, nullptr, &nr));
@@ -269,22 +277,18 @@ int main(int argc, char** argv) {
   int64_t skipped = 0;
   std::vector<int32_t> run;
   for (int32_t s : sources) {
-    bool zero = g->off[s + 1] == g->off[s] || (heuristic == 1 && round[s] != 0);
+    bool zero = g->off[s + 1] == g->off[s] || (peel && round[s] != 0);
     if (zero) skipped++;
     else run.push_back(s);
   }
-  if (heuristic == 1 && !exact) run = sources;  // the engine applies the same rule when expanding
-  if (heuristic > 1) {
-    printf("REMOVED\n");
-    run.clear();
-  }
+  if (peel && !exact) run = sources;  // the engine applies the same rule when expanding
   printf("task 0: BC computation is running...\n");
   bcmi_run_stats st;
   memset(&st, 0, sizeof st);
   t0 = now_s();
   if (!run.empty())
     CK(bcmi_bc_run(ctx, run.data(), (int64_t)run.size(), mode,
-                   heuristic == 1 && !exact ? BCMI_SRC_EXPAND : BCMI_SRC_EXACT, &st));
+                   (peel && !exact ? BCMI_SRC_EXPAND : BCMI_SRC_EXACT) | (deg2 ? BCMI_SRC_DEG2 : 0), &st));
   double all_time = now_s() - t0;
   std::vector<double> bc(g->n);
   t0 = now_s();
@@ -358,7 +362,8 @@ int main(int argc, char** argv) {
            (unsigned)vskipped);
   }
   printf("task 0 BC GTEPS (forward+backward arcs): %lf\n", all_time > 0 ? st.te / all_time / 1e9 : 0.0);
-  if (heuristic == 1) printf("task 0 1-Degree reduction: %lf sec\n", redu_time);
+  if (peel) printf("task 0 1-Degree reduction: %lf sec\n", redu_time);
+  if (deg2) printf("task 0 2-Degree sources taken from their neighbours: %" PRId64 "\n", st.deg2_sources);
   printf("\n");
   bcmi_destroy(ctx);
   bcmi_graph_free(g);

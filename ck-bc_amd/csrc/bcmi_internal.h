@@ -62,6 +62,7 @@ struct Params {
   int trace = 0;            // diagnostic: per-level statistics on stderr (use with steps = 1)
   int graphs = 1;           // 1: the level loop replays captured graphs; 0: it launches the same
                             // kernels one by one (for tools that do not follow graph launches)
+  int deg2_slots = 8;       // 2-degree sources: forward states of their neighbours kept at once
   int ktime = 0;            // kernel timed on the device (wave entry/exit stamps) inside the level loop:
                             // 0 none, 1 batched sigma, 2 batched backward, 3 batched top-down,
                             // 4 batched bottom-up, 5 plain top-down, 6 plain bottom-up, 7 plain backward

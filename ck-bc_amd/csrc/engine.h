@@ -30,6 +30,8 @@ struct Ctl {
   int32_t push;       // batched: this level's path counts are pushed from the frontier (k_ms_push)
   int32_t bpush;      // batched: this backward level pulls nothing; the successor level pushes (k_ms_bpush)
   int32_t narrow;     // batched: path counts held in 32-bit rows
+  int32_t cnt_back;   // plain: the backward pass counts the DAG arcs (forward state not swept)
+  int32_t pad_;
   int64_t rec_count;  // records appended so far
   int64_t arcs_next;  // degree sum of vertices appended during this step
   int64_t verts_next;
@@ -146,6 +148,7 @@ struct RunResult {
   int64_t pairs = 0;                  // batched: (vertex, depth) pairs
   int64_t scaled_batches = 0;         // batches (or workgroup sweeps) run with scaled FP64 path counts
   int64_t wg_sources = 0;             // sources run one per workgroup (deep graphs)
+  int64_t deg2_sources = 0;           // plain: sweeps whose forward state came from two neighbours'
   std::vector<int64_t> uni_n, uni_m;  // batched: union reach per batch
   double te = 0, bytes = 0, gpu_ms = 0, wall_ms = 0;
   double ktime_ms = 0;      // summed duration of the timed kernel (Params::ktime)

@@ -37,10 +37,13 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
 #include <thread>
+#include <unordered_map>
 
 #include "bcmi.h"
 #include "bcmi_internal.h"
@@ -578,6 +581,7 @@ __global__ void k_plain_init(GA g, PA p) {
   c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 1; c->uni_arcs = deg;
   c->sigma_ovf = 0;
   c->scaled = 0;
+  c->cnt_back = 0;
   p.rec_lo[0] = 0;
   for (int k = 0; k < nch && k < p.cap; k++) { p.rec_v[k] = s; p.rec_j[k] = k; }
   if (nch > p.cap) c->overflow = 1;
@@ -836,6 +840,11 @@ __device__ __forceinline__ void k_plain_back_body(GA g, PA p) {
   const int64_t gw = (int64_t)(blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tw = (int64_t)(gridDim.x * blockDim.x) >> 6;
   const double mult = p.multbuf[0];
+  // a sweep whose forward state was not swept (2-degree source) counts its
+  // shortest-path DAG arcs here: every one of them is a successor arc of a
+  // level >= 1, except the source's own
+  const bool cnt = c->cnt_back;
+  long long acnt = 0;
   // headers of 64 of this wave's records at once, as in k_plain_td
   for (int64_t k0 = 0; lo + gw + k0 * tw < hi; k0 += 64) {
     const int64_t rq = lo + gw + (k0 + lane) * tw;
@@ -865,7 +874,10 @@ __device__ __forceinline__ void k_plain_back_body(GA g, PA p) {
         int64_t e = eb + lane;
         if (e < e1) {
           int32_t z = adj_at(g, g.adj, e);
-          if (p.depth[z] == d + 1) acc = acc + p.coef[z] * sv;
+          if (p.depth[z] == d + 1) {
+            acc = acc + p.coef[z] * sv;
+            acnt++;
+          }
         }
       }
       acc = wave_sum_d(acc);
@@ -880,6 +892,10 @@ __device__ __forceinline__ void k_plain_back_body(GA g, PA p) {
         }
       }
     }
+  }
+  if (cnt) {
+    acnt = wave_sum_ll(acnt);
+    if (lane_id() == 0 && acnt) atomicAdd((u64*)&c->sum_a, (u64)acnt);
   }
 }
 
@@ -918,6 +934,172 @@ __global__ void k_plain_cleanup(PA p) {
       p.depth[v] = -1;
       p.sigma[v] = 0;
     }
+  }
+}
+
+// ============================================ 2-degree sources (plain sweeps)
+// A source v0 of degree 2 whose neighbours a and b are swept too: every
+// shortest path out of v0 leaves through a or b, so for t != v0
+//   d0(t)     = 1 + min(da(t), db(t))
+//   sigma0(t) = [da(t) = d0(t)-1] sigma_a(t) + [db(t) = d0(t)-1] sigma_b(t)
+// (a shortest path from a to t through v0 is 2 longer than b's distance, so
+// it is never counted on a minimal side).  v0's forward state is one pass
+// over the saved depths and path counts of a's and b's sweeps, its level
+// lists come from a counting sort on depth, and its backward pass is the
+// plain one, which also counts the DAG arcs that no forward pass counted.
+// The reference declares this heuristic (setcuda_2degree h:133,
+// set_mlp_cuda_2degree h:137, scan_frt_csc_cuda_mono_2degree h:147,
+// update_bc_cuda_2degree h:156) and prints "REMOVED" for it (c:1015).
+#define D2_LDS_LEVELS 64
+
+struct D2A {
+  const int32_t* da;  // neighbour a's saved depths and path counts
+  const u64* sa;
+  const int32_t* db;  // neighbour b's
+  const u64* sb;
+  int32_t v0;
+  int64_t* lcnt;      // [level] records of the level, then its write cursor
+  int64_t nlev;       // levels lcnt holds: no depth reaches it (min of the two sweeps' levels + 2)
+};
+
+__global__ void k_d2_init(GA g, PA p, D2A d) {
+  Ctl* c = p.ctl;
+  c->level = 0; c->started = 1; c->fdone = 0; c->last = 0;
+  c->blevel = 0; c->bstarted = 0; c->bdone = 0; c->mode = 0;
+  c->overflow = 0; c->sigma_ovf = 0; c->scaled = 0;
+  c->cnt_back = 1;
+  c->src_label = g.label[d.v0];
+  c->rec_count = 0;
+  c->arcs_next = 0; c->verts_next = 0; c->arcs_cur = 0; c->verts_cur = 0; c->arcs_seen = 0;
+  c->arcs_reach = g.comp_arcs[g.label[d.v0]];
+  c->sum_a = 2;  // v0's own arcs, to a and b on level 1
+  c->n_td = 0; c->n_bu = 0;
+  c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 0; c->uni_arcs = 0;
+  for (int64_t i = 0; i < d.nlev; i++) d.lcnt[i] = 0;
+}
+
+// Depths, path counts and delta = w(t) of every vertex v0 reaches, records
+// per level (block histograms in LDS), reached vertices and their degree sum.
+__global__ void __launch_bounds__(256) k_d2_combine(GA g, PA p, D2A d) {
+  __shared__ unsigned long long h[D2_LDS_LEVELS];
+  __shared__ int s_max, s_flags;
+  for (int i = threadIdx.x; i < D2_LDS_LEVELS; i += blockDim.x) h[i] = 0;
+  if (threadIdx.x == 0) { s_max = 0; s_flags = 0; }
+  __syncthreads();
+  Ctl* c = p.ctl;
+  long long verts = 0, arcs = 0;
+  int ovf = 0, bad = 0, dmax = 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int x = d.da[t], y = d.db[t];
+    if (x < 0 && y < 0) continue;
+    int dep;
+    u64 sg;
+    if (t == d.v0) {
+      dep = 0;
+      sg = 1;
+    } else {
+      const int mn = x < 0 ? y : y < 0 ? x : min(x, y);
+      dep = mn + 1;
+      sg = x == mn ? d.sa[t] : 0;
+      if (y == mn) {
+        const u64 s2 = sg + d.sb[t];
+        if (s2 < sg) ovf = 1;
+        sg = s2;
+      }
+    }
+    if (dep + 1 >= d.nlev) {
+      bad = 1;
+      continue;
+    }
+    p.depth[t] = dep;
+    p.sigma[t] = sg;
+    p.delta[t] = g.w ? (double)g.w[t] : 0.0;
+    const int64_t deg = g.off[t + 1] - g.off[t];
+    const int nch = nchunks(deg, g.chunk);
+    verts++;
+    arcs += deg;
+    dmax = max(dmax, dep);
+    if (dep < D2_LDS_LEVELS) atomicAdd(&h[dep], (unsigned long long)nch);
+    else atomicAdd((u64*)&d.lcnt[dep], (u64)nch);
+  }
+  // one global update per block for each total (same-address atomics serialise)
+  if (dmax) atomicMax(&s_max, dmax);
+  if (ovf | bad) atomicOr(&s_flags, ovf | (bad << 1));
+  __syncthreads();
+  for (int i = threadIdx.x; i < D2_LDS_LEVELS && i < d.nlev; i += blockDim.x)
+    if (h[i]) atomicAdd((u64*)&d.lcnt[i], (u64)h[i]);
+  block_add_uniform(wave_sum_ll(verts), (u64*)&c->tot_verts);
+  block_add_uniform(wave_sum_ll(arcs), (u64*)&c->tot_arcs);
+  if (threadIdx.x == 0) {
+    if (s_max) atomicMax(&c->last, s_max);
+    if (s_flags & 1) c->sigma_ovf = 1;
+    if (s_flags & 2) c->overflow = 1;
+  }
+}
+
+// Level offsets (rec_lo) and write cursors from the per-level record counts.
+__global__ void k_d2_scan(PA p, D2A d) {
+  Ctl* c = p.ctl;
+  const int last = c->last;
+  int64_t acc = 0;
+  for (int l = 0; l <= last; l++) {
+    const int64_t k = d.lcnt[l];
+    p.rec_lo[l] = acc;
+    d.lcnt[l] = acc;
+    acc += k;
+  }
+  p.rec_lo[last + 1] = acc;
+  c->rec_count = acc;
+  if (acc > p.cap) c->overflow = 1;
+  c->fdone = 1;
+}
+
+// Writes every reached vertex's records into its level.  Each block counts
+// its vertices' records per level, reserves one range per level, then hands
+// out positions in the range with LDS atomics (the order inside a level does
+// not matter to the backward pass).
+__global__ void __launch_bounds__(256) k_d2_scatter(GA g, PA p, D2A d) {
+  if (p.ctl->overflow) return;
+  __shared__ unsigned long long h[D2_LDS_LEVELS];
+  for (int i = threadIdx.x; i < D2_LDS_LEVELS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, ts = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = t0; t < g.n; t += ts) {
+    const int dep = p.depth[t];
+    if (dep >= 0 && dep < D2_LDS_LEVELS) atomicAdd(&h[dep], (unsigned long long)nchunks(g.off[t + 1] - g.off[t], g.chunk));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < D2_LDS_LEVELS && i < d.nlev; i += blockDim.x)
+    if (h[i]) h[i] = atomicAdd((u64*)&d.lcnt[i], (u64)h[i]);
+  __syncthreads();
+  for (int64_t t = t0; t < g.n; t += ts) {
+    const int dep = p.depth[t];
+    if (dep < 0) continue;
+    const int nch = nchunks(g.off[t + 1] - g.off[t], g.chunk);
+    const int64_t pos = dep < D2_LDS_LEVELS ? (int64_t)atomicAdd(&h[dep], (unsigned long long)nch)
+                                            : (int64_t)atomicAdd((u64*)&d.lcnt[dep], (u64)nch);
+    for (int k = 0; k < nch; k++) {
+      p.rec_v[pos + k] = (int32_t)t;
+      p.rec_j[pos + k] = k;
+    }
+  }
+}
+
+// Undoes k_d2_combine's stores (a combined state that cannot be used).
+__global__ void k_d2_reset(GA g, PA p, D2A d) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.n; t += (int64_t)gridDim.x * blockDim.x) {
+    if (d.da[t] < 0 && d.db[t] < 0) continue;
+    p.depth[t] = -1;
+    p.sigma[t] = 0;
+  }
+}
+
+// The two neighbours of each listed vertex of degree 2.
+__global__ void k_d2_nb(const int64_t* off, const int32_t* adj, const int32_t* v, int64_t k, int32_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = off[v[i]];
+    out[2 * i] = adj[o];
+    out[2 * i + 1] = adj[o + 1];
   }
 }
 
@@ -2850,6 +3032,13 @@ struct Workspace {
   int64_t* wg_st = nullptr;           // 3 x wg_cap: n, m, a
   int32_t* wg_st2 = nullptr;          // 2 x wg_cap: levels, flag
   size_t wg_bytes = 0;
+  // 2-degree sources (plain): saved forward states of their neighbours and
+  // the per-level record counts of a combined state
+  int32_t* d2_depth = nullptr;        // d2_slots x n
+  u64* d2_sigma = nullptr;            // d2_slots x n
+  int d2_slots = 0;
+  int64_t* d2_lcnt = nullptr;
+  int64_t d2_lcap = 0;
   const DevGraph* cap_graph = nullptr;
   size_t bytes = 0;
 };
@@ -2950,6 +3139,7 @@ void workspace_destroy(Workspace* ws) {
   fr(ws->srcbuf); fr(ws->multbuf); fr(ws->depth); fr(ws->sigma1); fr(ws->delta1); fr(ws->coef1);
   fr(ws->seen); fr(ws->lm); fr(ws->lb); fr(ws->lscale); fr(ws->lvmax); fr(ws->sigma); fr(ws->coef); fr(ws->compmask); fr(ws->stats); fr(ws->dbg_delta); fr(ws->lvl_arcs);
   fr(ws->cnt0); fr(ws->cntS); fr(ws->sa); fr(ws->srcK); fr(ws->srcdev);
+  fr(ws->d2_depth); fr(ws->d2_sigma); fr(ws->d2_lcnt);
   if (ws->hctl) (void)hipHostFree(ws->hctl);
   if (ws->hkt) (void)hipHostFree(ws->hkt);
   if (ws->kt) (void)hipFree(ws->kt);
@@ -3228,6 +3418,29 @@ static int replay(const DevGraph& g, Workspace* ws, const Params& prm, int which
   return BCMI_OK;
 }
 
+// Backward levels of a sweep whose forward pass has finished (the host
+// mirror of the control block is current), replayed until the control block
+// reports the pass done.
+static int backward_loop(const DevGraph& g, Workspace* ws, const Params& prm) {
+  if (ws->hctl->bdone) return BCMI_OK;
+  hipStream_t st = ws->st;
+  // pick the grid of each backward replay from the sizes of its levels
+  int last = ws->hctl->last;
+  std::vector<int64_t> lo(last + 2);
+  HIP_TRY(hipMemcpyAsync(lo.data(), ws->rec_lo, sizeof(int64_t) * (last + 2), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  while (!ws->hctl->bdone) {
+    int top = ws->hctl->bstarted ? ws->hctl->blevel - 1 : last;
+    int bot = std::max(0, top - prm.steps + 1);
+    int64_t recs = lo[std::min(top + 1, last + 1)] - lo[bot];
+    bool sparse = recs < prm.sparse_verts * (int64_t)(top - bot + 1);
+    TRY(replay(g, ws, prm, sparse ? 3 : 1));
+    HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  return BCMI_OK;
+}
+
 // Runs one sweep (plain: srcbuf[0]; batched: srcbuf[0..k)) whose inputs are
 // already in the workspace buffers.  keep != 0 skips the cleanup (debug
 // readback), the caller then calls sweep_cleanup.
@@ -3315,22 +3528,7 @@ static int sweep(const DevGraph& g, Workspace* ws, const Params& prm, int k, boo
       if (prm.graphs) TRY(capture(g, ws, prm));
       continue;
     }
-    if (!ws->hctl->bdone) {
-      // pick the grid of each backward replay from the sizes of its levels
-      int last = ws->hctl->last;
-      std::vector<int64_t> lo(last + 2);
-      HIP_TRY(hipMemcpyAsync(lo.data(), ws->rec_lo, sizeof(int64_t) * (last + 2), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-      while (!ws->hctl->bdone) {
-        int top = ws->hctl->bstarted ? ws->hctl->blevel - 1 : last;
-        int bot = std::max(0, top - prm.steps + 1);
-        int64_t recs = lo[std::min(top + 1, last + 1)] - lo[bot];
-        bool sparse = recs < prm.sparse_verts * (int64_t)(top - bot + 1);
-        TRY(replay(g, ws, prm, sparse ? 3 : 1));
-        HIP_TRY(hipMemcpyAsync(ws->hctl, ws->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-      }
-    }
+    TRY(backward_loop(g, ws, prm));
     break;
   }
   if (levels) *levels = ws->hctl->last + 1;
@@ -3425,16 +3623,9 @@ struct RunPart {
 };
 
 // One plain sweep from source s (internal id) with BC multiplier mult.
-static int run_plain_one(const DevGraph& g, Workspace* w, const Params& prm, int32_t s, double mult, RunPart& P) {
-  const double t0 = now_ms();
-  hipStream_t st = w->st;
-  w->hsrc[0] = s;
-  w->hmult[0] = mult;
-  HIP_TRY(hipMemcpyAsync(w->srcbuf, w->hsrc, sizeof(int32_t), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(w->multbuf, w->hmult, sizeof(double), hipMemcpyHostToDevice, st));
-  int lv = 0;
-  TRY(sweep(g, w, prm, 1, false, &lv));
-  const Ctl& c = *w->hctl;
+// Records the statistics of the plain sweep just finished (host mirror of
+// its control block) that took t_ms and reached `lv` levels.
+static void plain_stats(const Ctl& c, int lv, double t_ms, RunPart& P) {
   RunResult& R = P.R;
   R.n_s.push_back(c.tot_verts);
   R.m_s.push_back(c.tot_arcs);
@@ -3443,8 +3634,267 @@ static int run_plain_one(const DevGraph& g, Workspace* w, const Params& prm, int
   R.sweeps++; R.launches++;
   R.max_levels = std::max<int64_t>(R.max_levels, lv);
   R.levels.push_back(lv);
-  R.t_s.push_back(now_ms() - t0);
+  R.t_s.push_back(t_ms);
   if (c.sigma_ovf) P.ovf = 1;
+}
+
+// One plain sweep from source s (internal id) with BC multiplier mult.  With
+// keep the sweep's depths and path counts stay in the workspace (the caller
+// saves them, then calls sweep_cleanup); *levels gets its level count.
+static int run_plain_one(const DevGraph& g, Workspace* w, const Params& prm, int32_t s, double mult, RunPart& P,
+                         bool keep = false, int* levels = nullptr) {
+  const double t0 = now_ms();
+  hipStream_t st = w->st;
+  w->hsrc[0] = s;
+  w->hmult[0] = mult;
+  HIP_TRY(hipMemcpyAsync(w->srcbuf, w->hsrc, sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(w->multbuf, w->hmult, sizeof(double), hipMemcpyHostToDevice, st));
+  int lv = 0;
+  TRY(sweep(g, w, prm, 1, keep, &lv));
+  if (levels) *levels = lv;
+  plain_stats(*w->hctl, lv, now_ms() - t0, P);
+  return BCMI_OK;
+}
+
+// Saved forward states of 2-degree sources' neighbours: slot i holds depths
+// at d2_depth + i*n and path counts at d2_sigma + i*n.
+static int d2_reserve(Workspace* ws, int slots, int64_t nlev) {
+  ALLOC_LOCK;
+  auto fr = [](void* p) { if (p) (void)hipFree(p); };
+  if (slots > ws->d2_slots) {
+    fr(ws->d2_depth); fr(ws->d2_sigma);
+    ws->d2_depth = nullptr; ws->d2_sigma = nullptr; ws->d2_slots = 0;
+    TRY(dalloc(&ws->d2_depth, (size_t)slots * ws->n, &ws->bytes));
+    TRY(dalloc(&ws->d2_sigma, (size_t)slots * ws->n, &ws->bytes));
+    ws->d2_slots = slots;
+  }
+  if (nlev > ws->d2_lcap) {
+    fr(ws->d2_lcnt);
+    ws->d2_lcnt = nullptr; ws->d2_lcap = 0;
+    const int64_t cap = std::max<int64_t>(nlev, 256);
+    TRY(dalloc(&ws->d2_lcnt, (size_t)cap, &ws->bytes));
+    ws->d2_lcap = cap;
+  }
+  return BCMI_OK;
+}
+
+// Plain sweep of a 2-degree source v0 whose neighbours' forward states sit
+// in slots sa and sb (lva, lvb levels): the forward state is combined from
+// them (k_d2_*), then the plain backward pass runs.  *done is false when the
+// combined state could not be used (it is undone; v0 is then swept normally).
+static int run_plain_derived(const DevGraph& g, Workspace* w, const Params& prm, int32_t v0, double mult, int sa,
+                             int sb, int lva, int lvb, RunPart& P, bool* done) {
+  const double t0 = now_ms();
+  hipStream_t st = w->st;
+  const int64_t n = g.n;
+  const int64_t nlev = (int64_t)std::min(lva, lvb) + 2;
+  *done = false;
+  TRY(d2_reserve(w, w->d2_slots, nlev));
+  if (prm.graphs && (!w->gfwd || w->cap_graph != &g || !same_launch(w->cap_prm, prm))) TRY(capture(g, w, prm));
+  w->hmult[0] = mult;
+  HIP_TRY(hipMemcpyAsync(w->multbuf, w->hmult, sizeof(double), hipMemcpyHostToDevice, st));
+  GA ga = make_ga(g, prm);
+  PA pa = make_pa(w, prm);
+  D2A d;
+  d.da = w->d2_depth + (size_t)sa * n; d.sa = w->d2_sigma + (size_t)sa * n;
+  d.db = w->d2_depth + (size_t)sb * n; d.sb = w->d2_sigma + (size_t)sb * n;
+  d.v0 = v0; d.lcnt = w->d2_lcnt; d.nlev = nlev;
+  const int G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+  hipLaunchKernelGGL(k_d2_init, 1, 1, 0, st, ga, pa, d);
+  hipLaunchKernelGGL(k_d2_combine, G, 256, 0, st, ga, pa, d);
+  hipLaunchKernelGGL(k_d2_scan, 1, 1, 0, st, pa, d);
+  hipLaunchKernelGGL(k_d2_scatter, G, 256, 0, st, ga, pa, d);
+  HIP_TRY(hipMemcpyAsync(w->hctl, w->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (w->hctl->overflow) {
+    hipLaunchKernelGGL(k_d2_reset, G, 256, 0, st, ga, pa, d);
+    HIP_TRY(hipStreamSynchronize(st));
+    return BCMI_OK;
+  }
+  TRY(backward_loop(g, w, prm));
+  const int lv = w->hctl->last + 1;
+  hipLaunchKernelGGL(k_plain_cleanup, grid_for(std::min<int64_t>(w->hctl->rec_count, w->cap)), 256, 0, st, pa);
+  HIP_TRY(hipStreamSynchronize(st));
+  plain_stats(*w->hctl, lv, now_ms() - t0, P);
+  P.R.deg2_sources++;
+  *done = true;
+  return BCMI_OK;
+}
+
+// Plain sweeps with the 2-degree heuristic (BCMI_SRC_DEG2).  A listed vertex
+// v0 of traversed degree 2 whose neighbours a and b are listed is taken from
+// them when neither neighbour is itself taken that way and v0 is nobody's
+// neighbour-to-be-saved (first come, in increasing id order); a and b become
+// anchors.  Anchors run first, ordered so that a pair's anchors run close
+// together (breadth-first over the pairs, from the anchors with most pairs);
+// after an anchor's sweep its forward state goes into a slot, and every pair
+// whose other anchor is in a slot runs its derived sources.  Slots are reused
+// least recently used first, preferring anchors with no pairs left; a derived
+// source whose anchor left the slots before its partner ran is swept
+// normally, as is every other listed source.
+static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, const std::vector<int32_t>& cs,
+                          const std::vector<double>& cm, RunPart& P) {
+  const int64_t n = g.n;
+  const std::vector<int64_t>& ho = g.h_off;
+  hipStream_t st = w->st;
+  std::vector<int32_t> uniq(cs);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  auto listed = [&](int32_t v) { return std::binary_search(uniq.begin(), uniq.end(), v); };
+  std::vector<int32_t> cand;
+  for (int32_t v : uniq)
+    if (ho[v + 1] - ho[v] == 2) cand.push_back(v);
+  std::vector<int32_t> nb(2 * cand.size());
+  if (!cand.empty()) {
+    int32_t *dc = nullptr, *dn = nullptr;
+    {
+      ALLOC_LOCK;
+      TRY(dalloc(&dc, cand.size()));
+      if (dalloc(&dn, 2 * cand.size()) != BCMI_OK) { (void)hipFree(dc); return BCMI_ERR_NOMEM; }
+    }
+    hipError_t e = hipMemcpyAsync(dc, cand.data(), sizeof(int32_t) * cand.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_d2_nb, grid_for((int64_t)cand.size()), 256, 0, st, g.off, g.adj, dc, (int64_t)cand.size(), dn);
+      e = hipMemcpyAsync(nb.data(), dn, sizeof(int32_t) * nb.size(), hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    {
+      ALLOC_LOCK;
+      (void)hipFree(dc);
+      (void)hipFree(dn);
+    }
+    if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+  }
+  // roles: 1 anchor, 2 derived; pairs of anchors with the derived vertices they serve
+  std::vector<uint8_t> role(n, 0);
+  struct Pair {
+    int32_t a, b;
+    std::vector<size_t> items;  // indices into cs of the derived sources
+    bool done = false;
+  };
+  std::vector<Pair> pairs;
+  std::map<std::pair<int32_t, int32_t>, int> pid;
+  std::unordered_map<int32_t, int> pair_of;  // derived vertex -> pair
+  for (size_t i = 0; i < cand.size(); i++) {
+    const int32_t v0 = cand[i], a = nb[2 * i], b = nb[2 * i + 1];
+    if (role[v0] || a == b || role[a] == 2 || role[b] == 2 || !listed(a) || !listed(b)) continue;
+    role[v0] = 2;
+    role[a] = role[b] = 1;
+    auto key = std::make_pair(std::min(a, b), std::max(a, b));
+    auto it = pid.find(key);
+    if (it == pid.end()) {
+      it = pid.emplace(key, (int)pairs.size()).first;
+      pairs.push_back(Pair{key.first, key.second, {}, false});
+    }
+    pair_of[v0] = it->second;
+  }
+  std::unordered_map<int32_t, std::vector<int>> pairs_at;   // anchor -> its pairs
+  std::unordered_map<int32_t, size_t> anchor_item;          // anchor -> its first listed occurrence
+  for (size_t i = 0; i < cs.size(); i++) {
+    const int32_t v = cs[i];
+    if (role[v] == 2) pairs[pair_of[v]].items.push_back(i);
+    else if (role[v] == 1 && !anchor_item.count(v)) anchor_item[v] = i;
+  }
+  for (size_t q = 0; q < pairs.size(); q++) {
+    pairs_at[pairs[q].a].push_back((int)q);
+    pairs_at[pairs[q].b].push_back((int)q);
+  }
+  // anchor order: breadth-first over the pairs, starting from the anchors with most pairs
+  std::vector<int32_t> anchors;
+  for (auto& kv : pairs_at) anchors.push_back(kv.first);
+  std::sort(anchors.begin(), anchors.end(), [&](int32_t x, int32_t y) {
+    size_t px = pairs_at[x].size(), py = pairs_at[y].size();
+    return px != py ? px > py : x < y;
+  });
+  std::vector<int32_t> order;
+  {
+    std::unordered_map<int32_t, char> seen;
+    for (int32_t r : anchors) {
+      if (seen.count(r)) continue;
+      std::deque<int32_t> q{r};
+      seen[r] = 1;
+      while (!q.empty()) {
+        const int32_t x = q.front();
+        q.pop_front();
+        order.push_back(x);
+        std::vector<int32_t> nx;
+        for (int pq : pairs_at[x]) {
+          const int32_t y = pairs[pq].a == x ? pairs[pq].b : pairs[pq].a;
+          if (!seen.count(y)) { seen[y] = 1; nx.push_back(y); }
+        }
+        std::sort(nx.begin(), nx.end(), [&](int32_t u, int32_t v) {
+          size_t pu = pairs_at[u].size(), pv = pairs_at[v].size();
+          return pu != pv ? pu > pv : u < v;
+        });
+        for (int32_t y : nx) q.push_back(y);
+      }
+    }
+  }
+  std::vector<uint8_t> ran(cs.size(), 0);
+  const int S = (int)std::min<int64_t>(prm.deg2_slots, (int64_t)order.size());
+  if (S >= 2) {
+    TRY(d2_reserve(w, S, 256));
+    std::vector<int32_t> slot_v(S, -1);
+    std::vector<int> slot_lv(S, 0);
+    std::vector<int64_t> slot_t(S, 0);
+    std::unordered_map<int32_t, int> slot_of;
+    std::unordered_map<int32_t, int> pending;
+    for (auto& kv : pairs_at) pending[kv.first] = (int)kv.second.size();
+    int64_t clock = 0;
+    for (int32_t a : order) {
+      const size_t ia = anchor_item[a];
+      int lv = 0;
+      TRY(run_plain_one(g, w, prm, a, cm[ia], P, true, &lv));
+      ran[ia] = 1;
+      const bool usable = !w->hctl->sigma_ovf;
+      if (usable) {
+        // a free slot, else one whose anchor has no pairs left, else the least
+        // recently used one not holding a partner a is about to pair with
+        int s = -1;
+        for (int i = 0; i < S && s < 0; i++)
+          if (slot_v[i] < 0) s = i;
+        for (int i = 0; i < S && s < 0; i++)
+          if (pending[slot_v[i]] == 0) s = i;
+        if (s < 0) {
+          auto partner = [&](int32_t x) {
+            for (int pq : pairs_at[a])
+              if (!pairs[pq].done && (pairs[pq].a == x || pairs[pq].b == x)) return true;
+            return false;
+          };
+          for (int pass = 0; pass < 2 && s < 0; pass++)
+            for (int i = 0; i < S; i++)
+              if ((pass == 1 || !partner(slot_v[i])) && (s < 0 || slot_t[i] < slot_t[s])) s = i;
+        }
+        if (slot_v[s] >= 0) slot_of.erase(slot_v[s]);
+        HIP_TRY(hipMemcpyAsync(w->d2_depth + (size_t)s * n, w->depth, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(w->d2_sigma + (size_t)s * n, w->sigma1, sizeof(u64) * n, hipMemcpyDeviceToDevice, st));
+        slot_v[s] = a;
+        slot_lv[s] = lv;
+        slot_t[s] = ++clock;
+        slot_of[a] = s;
+      }
+      TRY(sweep_cleanup(g, w, prm));
+      if (!usable) continue;
+      for (int pq : pairs_at[a]) {
+        Pair& pr = pairs[pq];
+        const int32_t b = pr.a == a ? pr.b : pr.a;
+        if (pr.done || !slot_of.count(b)) continue;
+        const int sa = slot_of[a], sb = slot_of[b];
+        for (size_t j : pr.items) {
+          bool ok = false;
+          TRY(run_plain_derived(g, w, prm, cs[j], cm[j], sa, sb, slot_lv[sa], slot_lv[sb], P, &ok));
+          if (ok) ran[j] = 1;
+        }
+        pr.done = true;
+        pending[a]--;
+        pending[b]--;
+        slot_t[sb] = ++clock;
+        slot_t[sa] = ++clock;
+      }
+    }
+  }
+  for (size_t i = 0; i < cs.size(); i++)
+    if (!ran[i]) TRY(run_plain_one(g, w, prm, cs[i], cm[i], P));
   return BCMI_OK;
 }
 
@@ -3794,6 +4244,8 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   const size_t nitems = wg ? 0 : plain ? cs.size() : (cs.size() + W - 1) / W;
   const int K = (int)std::min<size_t>((size_t)concurrent_sweeps(g, prm), std::max<size_t>(1, nitems));
   if (K > 1) TRY(ensure_helpers(g, ws, prm, K - 1));
+  // 2-degree sources: plain sweeps on one workspace of an undirected graph
+  const bool deg2 = (src_flags & BCMI_SRC_DEG2) && plain && !wg && K == 1 && !g.directed && prm.deg2_slots >= 2;
   std::vector<RunPart> parts(K);
   std::atomic<size_t> next{0};
   std::atomic<bool> stop{false};
@@ -3808,6 +4260,15 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
       return;
     }
     Workspace* w = t == 0 ? ws : ws->helpers[t - 1];
+    if (deg2) {
+      // one workspace (K == 1): the derived sources need their neighbours' saved states
+      int rc = run_plain_deg2(g, w, prm, cs, cm, P);
+      if (rc != BCMI_OK) {
+        P.rc = rc;
+        P.msg = bcmi_last_error();
+      }
+      return;
+    }
     while (!stop) {
       size_t it = next.fetch_add(1);
       if (it >= nitems) break;
@@ -3847,6 +4308,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
     R.n_td += Q.n_td; R.n_bu += Q.n_bu;
     R.sweeps += Q.sweeps; R.launches += Q.launches;
     R.scaled_batches += Q.scaled_batches;
+    R.deg2_sources += Q.deg2_sources;
     R.max_levels = std::max(R.max_levels, Q.max_levels);
     ovf |= P.ovf;
   }

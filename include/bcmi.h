@@ -46,6 +46,15 @@ extern "C" {
 #define BCMI_SRC_EXACT 0    /* BC summed over exactly the listed sources */
 #define BCMI_SRC_EXPAND 1   /* reference semantics: a listed core vertex stands for its whole
                                pruned tree, listed pruned vertices are skipped (c:1026, cu:274) */
+/* 2-degree heuristic (-H 2, and -H 3 with peeling; declared but never defined
+ * in the reference: setcuda_2degree h:133, set_mlp_cuda_2degree h:137,
+ * scan_frt_csc_cuda_mono_2degree h:147, update_bc_cuda_2degree h:156, c:759-760,
+ * c:1015).  OR-ed into src_flags: in BCMI_MODE_PLAIN a listed source of
+ * (traversed) degree 2 whose two neighbours are listed too takes its depths
+ * and path counts from the saved forward states of their sweeps instead of a
+ * forward sweep of its own; BC is unchanged.  Batched runs, directed graphs,
+ * deep graphs (workgroup sweeps) and sweeps on several streams ignore it. */
+#define BCMI_SRC_DEG2 2
 
 /* Host graph in CSR form: adjacency of v = off[v]..off[v+1] in adj, sorted,
  * no duplicates, no self loops.  For directed graphs adj lists in-neighbours
@@ -76,6 +85,7 @@ typedef struct bcmi_run_stats {
     int64_t sum_union;     /* batched: distinct vertices reached per batch, summed over batches */
     int64_t scaled_batches; /* batches (or workgroup sweeps) run with scaled FP64 path counts */
     int64_t wg_sources;    /* sources run one per workgroup (deep graphs, see "strategy") */
+    int64_t deg2_sources;  /* plain sweeps whose forward state came from two neighbours' (BCMI_SRC_DEG2) */
 } bcmi_run_stats;
 
 /* ---- error reporting ---------------------------------------------------- */
@@ -155,7 +165,9 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * per source: deep graphs, whose hub BFS depth reaches "wg_depth", run
  * hundreds of sources at once, one sweep per workgroup, within "wg_mem"
  * bytes of per-source state and at most "wg_slots" sources in flight),
- * "trace", "graphs" (1, the default: the level loop replays captured
+ * "deg2_slots" (forward states of 2-degree sources' neighbours held at once
+ * under BCMI_SRC_DEG2, 12 bytes per vertex each; default 8), "trace",
+ * "graphs" (1, the default: the level loop replays captured
  * graphs; 0: it launches the same kernels one by one, for profilers and
  * other tools that do not follow graph launches). */
 int bcmi_set_param(bcmi_ctx *ctx, const char *key, int64_t value);
@@ -183,8 +195,8 @@ int bcmi_bc_reset(bcmi_ctx *ctx);
  * accumulator (replaces the source loop c:1010-1050 with setcuda,
  * set_mlp_cuda cu:689-703, scan_col_csc_cuda_mono cu:835-918,
  * scan_frt_csc_cuda_mono cu:766-827, pre_update_bc_cuda cu:330-336 and
- * update_bc_cuda cu:280-286).  src_flags: BCMI_SRC_EXACT / BCMI_SRC_EXPAND.
- * Sources of degree 0 contribute nothing (c:1026-1035).  st may be NULL. */
+ * update_bc_cuda cu:280-286).  src_flags: BCMI_SRC_EXACT / BCMI_SRC_EXPAND,
+ * optionally OR-ed with BCMI_SRC_DEG2.  Sources of degree 0 contribute nothing (c:1026-1035).  st may be NULL. */
 int bcmi_bc_run(bcmi_ctx *ctx, const int32_t *sources, int64_t nsrc, int mode, int src_flags,
                 bcmi_run_stats *st);
 /* Allocation-free variant of bcmi_bc_run in BCMI_MODE_PLAIN.  No counterpart

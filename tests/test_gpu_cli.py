@@ -57,6 +57,26 @@ def test_ck_run_heuristic_full_precision(tmp_path):
     assert np.max(np.abs(got[nz] - want[nz]) / want[nz]) < 1e-9
 
 
+@pytest.mark.parametrize("heur", ["2", "3"])
+def test_ck_run_two_degree_heuristic(tmp_path, heur):
+    """-H 2 / -H 3 (the reference accepts them and prints REMOVED, bc2d.c:759-760,
+    c:1015): plain sweeps take degree-2 sources from their neighbours' sweeps
+    and the result file holds the same BC as networkx over all sources."""
+    fn, f = write_facebook(tmp_path)
+    out = tmp_path / "res"
+    r = subprocess.run([EXE, "-p", "1x1", "-f", str(fn), "-n", "4039", "-N", "4039", "-H", heur, "-B", "plain",
+                        "-P", "-o", str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "REMOVED" not in r.stdout
+    line = [l for l in r.stdout.splitlines() if "2-Degree sources taken from their neighbours" in l]
+    assert line and int(line[0].split()[-1]) > 0
+    got = read_result(str(out) + "_1X1_0.log")
+    want = f["bc_all"]
+    nz = want != 0
+    assert np.all(got[~nz] == 0)
+    assert np.max(np.abs(got[nz] - want[nz]) / want[nz]) < 1e-9
+
+
 def test_ck_run_tunables(tmp_path):
     """-T key=value reaches the engine: a launch shape of small records, few
     levels per captured graph and early bottom-up gives the same BC; an
