@@ -125,8 +125,11 @@ static int param_set(Params& p, const std::string& k, int64_t v, bool have_graph
     if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "wg_slots must be in [1,65536]");
     p.wg_slots = (int)v;
   } else if (k == "deg2_slots") {
-    if (v < 0 || v > 4096) return set_error(BCMI_ERR_ARG, "deg2_slots must be in [0,4096]");
+    if (v < 0 || v > (1 << 20)) return set_error(BCMI_ERR_ARG, "deg2_slots must be in [0,2^20]");
     p.deg2_slots = (int)v;
+  } else if (k == "deg2_mem") {
+    if (v < 0) return set_error(BCMI_ERR_ARG, "deg2_mem must be >= 0");
+    p.deg2_mem = v;
   } else if (k == "streams") {
     if (v < 1 || v > 16) return set_error(BCMI_ERR_ARG, "streams must be in [1,16]");
     p.streams = (int)v;
@@ -262,6 +265,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "wg_mem") *v = p.wg_mem;
   else if (k == "wg_slots") *v = p.wg_slots;
   else if (k == "deg2_slots") *v = p.deg2_slots;
+  else if (k == "deg2_mem") *v = p.deg2_mem;
   else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
   return BCMI_OK;
 }

@@ -62,7 +62,9 @@ struct Params {
   int trace = 0;            // diagnostic: per-level statistics on stderr (use with steps = 1)
   int graphs = 1;           // 1: the level loop replays captured graphs; 0: it launches the same
                             // kernels one by one (for tools that do not follow graph launches)
-  int deg2_slots = 8;       // 2-degree sources: forward states of their neighbours kept at once
+  int deg2_slots = 0;       // 2-degree sources: forward states of their neighbours kept at once
+                            // (0: as many as deg2_mem holds; 1 turns the heuristic off)
+  int64_t deg2_mem = 4ll << 30;  // device bytes for those states when deg2_slots is 0
   int ktime = 0;            // kernel timed on the device (wave entry/exit stamps) inside the level loop:
                             // 0 none, 1 batched sigma, 2 batched backward, 3 batched top-down,
                             // 4 batched bottom-up, 5 plain top-down, 6 plain bottom-up, 7 plain backward

@@ -3831,9 +3831,12 @@ static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, co
     }
   }
   std::vector<uint8_t> ran(cs.size(), 0);
-  const int S = (int)std::min<int64_t>(prm.deg2_slots, (int64_t)order.size());
+  int S = (int)std::min<int64_t>(prm.deg2_slots > 0 ? prm.deg2_slots
+                                                    : std::max<int64_t>(2, prm.deg2_mem / (12 * std::max<int64_t>(1, n))),
+                                 (int64_t)order.size());
+  // fewer slots while the device cannot hold them
+  while (S >= 2 && d2_reserve(w, S, 256) != BCMI_OK) S /= 2;
   if (S >= 2) {
-    TRY(d2_reserve(w, S, 256));
     std::vector<int32_t> slot_v(S, -1);
     std::vector<int> slot_lv(S, 0);
     std::vector<int64_t> slot_t(S, 0);
@@ -4245,7 +4248,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   const int K = (int)std::min<size_t>((size_t)concurrent_sweeps(g, prm), std::max<size_t>(1, nitems));
   if (K > 1) TRY(ensure_helpers(g, ws, prm, K - 1));
   // 2-degree sources: plain sweeps on one workspace of an undirected graph
-  const bool deg2 = (src_flags & BCMI_SRC_DEG2) && plain && !wg && K == 1 && !g.directed && prm.deg2_slots >= 2;
+  const bool deg2 = (src_flags & BCMI_SRC_DEG2) && plain && !wg && K == 1 && !g.directed && prm.deg2_slots != 1;
   std::vector<RunPart> parts(K);
   std::atomic<size_t> next{0};
   std::atomic<bool> stop{false};

@@ -166,7 +166,8 @@ int bcmi_destroy(bcmi_ctx *ctx);
  * hundreds of sources at once, one sweep per workgroup, within "wg_mem"
  * bytes of per-source state and at most "wg_slots" sources in flight),
  * "deg2_slots" (forward states of 2-degree sources' neighbours held at once
- * under BCMI_SRC_DEG2, 12 bytes per vertex each; default 8), "trace",
+ * under BCMI_SRC_DEG2, 12 bytes per vertex each; 0, the default: as many as
+ * "deg2_mem" bytes hold, 4 GiB by default; 1 turns the heuristic off), "trace",
  * "graphs" (1, the default: the level loop replays captured
  * graphs; 0: it launches the same kernels one by one, for profilers and
  * other tools that do not follow graph launches). */

@@ -89,19 +89,23 @@ def test_deg2_rmat12_all_sources_and_slots(engine):
     ref = O.brandes(g.off, g.adj, S, nthreads=8)
     derived = []
     try:
-        for slots in (8, 2, 64):
+        for slots in (8, 2, 0):   # 0: as many slots as the memory budget holds (all anchors here)
             engine.set_param("deg2_slots", slots)
             bc0, st0, bc1, st1 = runs(engine, S)
             assert bcmi.rel_err(bc1, ref) <= BC_TOL, slots
             same_totals(st0, st1)
             derived.append(st1["deg2_sources"])
-        engine.set_param("deg2_slots", 0)
+        engine.set_param("deg2_slots", 1)   # off
         _, _, bc1, st1 = runs(engine, S)
         assert st1["deg2_sources"] == 0
         assert bcmi.rel_err(bc1, ref) <= BC_TOL
     finally:
-        engine.set_param("deg2_slots", 8)
+        engine.set_param("deg2_slots", 0)
     assert min(derived) > 0, derived
+    # with every anchor in a slot, every degree-2 source the selection takes is derived
+    deg = np.diff(g.off)
+    assert derived[2] >= derived[0]
+    assert derived[2] >= 0.9 * np.count_nonzero(deg == 2), derived
 
 
 def test_deg2_duplicates_and_subsets(engine):
