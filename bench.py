@@ -25,6 +25,9 @@ Other workloads (not the default line):
   --scale 25 --sources 4096 --steps 62        R-MAT scale 25
   --graph chain --chain-blocks 32768 --sources 1024 --steps 12 --warmup 4 --kernel none --wg-mem 192
                                               bridged-block chain, 8.4M vertices
+  --deg2 [--graph facebook | --scale 16 [--pendant 0.3]]
+                                              2-degree heuristic: plain sweeps over every vertex as a
+                                              source, with and without it (-H 2; -H 3 when peeled)
 """
 import argparse
 import json
@@ -98,6 +101,52 @@ def serve_bench(args, log):
     return out
 
 
+def deg2_bench(args):
+    """Plain sweeps from every vertex of the graph (exact BC), once as they
+    are and once with the 2-degree heuristic (a degree-2 source whose two
+    neighbours are sources takes its forward state from their sweeps), each
+    timed after a warm run.  Prints one JSON line: both GTEPS (the same
+    traversed-edge count), the sources taken from their neighbours, the gain."""
+    import bcmi
+    if args.graph == "facebook":
+        f = np.load(os.path.join(ROOT, "tests", "golden", "facebook.npz"))
+        g = bcmi.Graph.from_edges(f["edges"], int(f["n"]))
+        gdesc = "facebook_combined (the reference's dataset)"
+    else:
+        g = bcmi.Graph.rmat(args.scale, args.edgefactor, seed=args.seed)
+        gdesc = f"R-MAT scale {args.scale} ef {args.edgefactor} (a=.57 b=.19 c=.19)"
+    if args.pendant > 0:
+        g = g.with_pendant_trees(args.pendant, seed=args.seed + 2)
+        gdesc += f" + pendant trees ({args.pendant:.0%} more vertices, peeled)"
+    peel = args.pendant > 0
+    e = bcmi.Engine(0)
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        e.set_param(k, int(v))
+    e.upload(g, heuristic=bcmi.H_PEEL if peel else bcmi.H_NONE)
+    src = np.arange(g.n, dtype=np.int32)
+    res = {}
+    for deg2 in (False, True):
+        e.run(src[:64], mode=bcmi.MODE_PLAIN, expand=peel, deg2=deg2)  # warm (capture, slots)
+        e.reset()
+        t0 = time.perf_counter()
+        st = e.run(src, mode=bcmi.MODE_PLAIN, expand=peel, deg2=deg2)
+        dt = time.perf_counter() - t0
+        res[deg2] = (dt, st)
+        log(f"deg2={deg2}: {dt:.3f}s, {st['te'] / dt / 1e9:.3f} GTEPS, {st['deg2_sources']} of "
+            f"{st['sources_run']} sweeps from their neighbours")
+    (t0_, s0), (t1_, s1) = res[False], res[True]
+    out = {"metric": "plain BC GTEPS with and without the 2-degree heuristic (all sources, 1 GPU)",
+           "value": round(s1["te"] / t1_ / 1e9, 3), "unit": "GTEPS", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "u64 sigma / f64 delta", "data": "synthetic, seeded" if args.graph != "facebook" else "reference dataset",
+           "config": {"workload": f"plain Brandes BC, every vertex a source ({'-H 3' if peel else '-H 2'}), {gdesc}",
+                      "n": g.n, "arcs": g.m},
+           "without_gteps": round(s0["te"] / t0_ / 1e9, 3), "sweeps": s1["sources_run"],
+           "from_neighbours": s1["deg2_sources"], "gain": round(t0_ / t1_ - 1.0, 4)}
+    print(json.dumps(out), flush=True)
+    e.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,8 +164,11 @@ def main():
     ap.add_argument("--kernel", default="back", choices=["back", "sigma", "td", "bu", "none"],
                     help="kernel timed on the device: back (the roofline kernel), sigma, td or bu "
                          "(time only), none (no device timer)")
-    ap.add_argument("--graph", default="rmat", choices=["rmat", "chain"],
-                    help="chain: bridged-block chain of --chain-blocks Erdos-Renyi blocks of 256 vertices, degree 8")
+    ap.add_argument("--graph", default="rmat", choices=["rmat", "chain", "facebook"],
+                    help="chain: bridged-block chain of --chain-blocks Erdos-Renyi blocks of 256 vertices, degree 8; "
+                         "facebook: the reference's dataset (the CK run's graph, from tests/golden; --deg2 only)")
+    ap.add_argument("--deg2", action="store_true",
+                    help="time plain sweeps over all sources with and without the 2-degree heuristic")
     ap.add_argument("--chain-blocks", type=int, default=32768)
     ap.add_argument("--wg-mem", type=int, default=0, metavar="GIB",
                     help="device memory for one-workgroup-per-source sweeps on deep graphs (0: engine default)")
@@ -151,6 +203,10 @@ def main():
         sys.exit(2)
 
     import bcmi
+
+    if args.deg2:
+        deg2_bench(args)
+        return
 
     t0 = time.time()
     if args.graph == "chain":
