@@ -564,6 +564,50 @@ __device__ __forceinline__ void block_add_uniform(long long v, u64* dst) {
   app_init(a, app_lv_, app_lj_)
 
 // ============================================================ plain sweep
+// Records of small vertices (all arcs in one record of at most PLAIN_PK) run
+// 64 / PLAIN_PK to a wave iteration, one sub-group of PLAIN_PK lanes each, so
+// that a vertex of a few arcs does not hold a whole wave through its chain
+// of dependent loads.
+#define PLAIN_PK 16
+static_assert(64 % PLAIN_PK == 0, "sub-groups tile the wave");
+// The record (header lane) of this lane's sub-group: sub-group k takes the
+// k-th of the next 64 / PLAIN_PK set bits of the wave-uniform mask P, which
+// loses them; -1 when P runs out.
+__device__ __forceinline__ int take_sub(u64& P) {
+  const int k = lane_id() / PLAIN_PK;
+  int q = -1;
+#pragma unroll
+  for (int i = 0; i < 64 / PLAIN_PK; i++) {
+    const int b = P ? __builtin_ctzll(P) : -1;
+    P &= P - 1;
+    if (k == i) q = b;
+  }
+  return q;
+}
+// Sum over a sub-group: the last steps of wave_sum_d's butterfly, so a record
+// whose terms sit on lanes 0 .. PLAIN_PK-1 gets the same additions in the
+// same order either way.
+__device__ __forceinline__ double sub_sum_d(double x) {
+#pragma unroll
+  for (int o = PLAIN_PK / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ long long sub_sum_ll(long long x) {
+#pragma unroll
+  for (int o = PLAIN_PK / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ u64 sub_sum_u64_ovf(u64 x, int* ovf) {
+#pragma unroll
+  for (int o = PLAIN_PK / 2; o > 0; o >>= 1) {
+    const u64 y = __shfl_xor(x, o, 64);
+    const u64 t = x + y;
+    if (t < x) *ovf = 1;
+    x = t;
+  }
+  return x;
+}
+
 __global__ void k_plain_init(GA g, PA p) {
   Ctl* c = p.ctl;
   int32_t s = p.srcbuf[0];
@@ -653,7 +697,38 @@ __device__ __forceinline__ void k_plain_td_body(GA g, PA p) {
       o1q = g.off[uq + 1];
       suq = p.sigma[uq];
     }
-    u64 R = __ballot(rq < hi);
+    // small frontier vertices first, 64 / PLAIN_PK at a time
+    u64 P = __ballot(rq < hi && o1q - o0q <= PLAIN_PK);
+    u64 R = __ballot(rq < hi) & ~P;
+    while (P) {
+      const int q = take_sub(P);
+      const int t = lane & (PLAIN_PK - 1), qs = q < 0 ? 0 : q;
+      const long long o0 = __shfl((long long)o0q, qs, 64);
+      const int dg = __shfl((int)(o1q - o0q), qs, 64);
+      const u64 su = __shfl((unsigned long long)suq, qs, 64);
+      bool won = false;
+      int64_t dgv = 0;
+      int32_t v = 0;
+      if (q >= 0 && t < dg) {
+        v = adj_at(g, g.adj, o0 + t);
+        int dv = p.depth[v];
+        if (dv == -1) {
+          const int old = atomicCAS(&p.depth[v], -1, d + 1);
+          won = old == -1;
+          dv = won ? d + 1 : old;
+        }
+        if (dv == d + 1) {
+          const u64 o = atomicAdd(&p.sigma[v], su);
+          if (o + su < o) ovf = 1;
+          acnt++;
+        }
+        if (won) {
+          p.delta[v] = g.w ? (double)g.w[v] : 0.0;
+          dgv = g.off[v + 1] - g.off[v];
+        }
+      }
+      app_push(ap, won, v, dgv, g.chunk, c, p.rec_v, p.rec_j, p.cap);
+    }
     while (R) {
       const int q = __ffsll((long long)R) - 1;
       R &= R - 1;
@@ -729,7 +804,43 @@ __device__ __forceinline__ void k_plain_bu_body(GA g, PA p) {
         open = dv == -1 || dv == d + 1;
       }
     }
-    u64 O = __ballot(open);
+    // small open vertices (one record of at most PLAIN_PK arcs, so unvisited)
+    // first, 64 / PLAIN_PK at a time
+    int64_t o0q = 0, o1q = 0;
+    if (open) {
+      o0q = g.off[vq];
+      o1q = g.off[vq + 1];
+    }
+    u64 P = __ballot(open && o1q - o0q <= PLAIN_PK);
+    u64 O = __ballot(open) & ~P;
+    while (P) {
+      const int q = take_sub(P);
+      const int t = lane & (PLAIN_PK - 1), qs = q < 0 ? 0 : q;
+      const int32_t v = __shfl(vq, qs, 64);
+      const long long o0 = __shfl((long long)o0q, qs, 64);
+      const int dg = __shfl((int)(o1q - o0q), qs, 64);
+      u64 acc = 0;
+      long long cn = 0;
+      if (q >= 0 && t < dg) {
+        const int32_t u = adj_at(g, g.adj, o0 + t);
+        if (p.depth[u] == d) {
+          acc = p.sigma[u];
+          cn = 1;
+        }
+      }
+      acc = sub_sum_u64_ovf(acc, &ovf);
+      cn = sub_sum_ll(cn);
+      bool won = false;
+      if (q >= 0 && t == 0 && cn > 0) {
+        // v's only record: the sole writer of its depth and path count in this step
+        p.depth[v] = d + 1;
+        p.sigma[v] = acc;
+        p.delta[v] = g.w ? (double)g.w[v] : 0.0;
+        won = true;
+        acnt += cn;
+      }
+      app_push(ap, won, v, won ? dg : 0, g.chunk, c, p.rec_v, p.rec_j, p.cap);
+    }
     while (O) {
       const int oq = __ffsll((long long)O) - 1;
       O &= O - 1;
@@ -858,7 +969,32 @@ __device__ __forceinline__ void k_plain_back_body(GA g, PA p) {
       o1q = g.off[vq + 1];
       sq = p.sigma[vq];
     }
-    u64 R = __ballot(rq < hi);
+    // small vertices first, 64 / PLAIN_PK at a time
+    u64 P = __ballot(rq < hi && o1q - o0q <= PLAIN_PK);
+    u64 R = __ballot(rq < hi) & ~P;
+    while (P) {
+      const int q = take_sub(P);
+      const int t = lane & (PLAIN_PK - 1), qs = q < 0 ? 0 : q;
+      const int32_t v = __shfl(vq, qs, 64);
+      const long long o0 = __shfl((long long)o0q, qs, 64);
+      const int dg = __shfl((int)(o1q - o0q), qs, 64);
+      const double sv = (double)__shfl((unsigned long long)sq, qs, 64);
+      double acc = 0.0;
+      if (q >= 0 && t < dg) {
+        const int32_t z = adj_at(g, g.adj, o0 + t);
+        if (p.depth[z] == d + 1) {
+          acc = acc + p.coef[z] * sv;
+          acnt++;
+        }
+      }
+      acc = sub_sum_d(acc);
+      if (q >= 0 && t == 0) {
+        const double dl = p.delta[v] + acc;
+        p.delta[v] = dl;
+        p.coef[v] = (dl + 1.0) / sv;
+        p.bc[v] += mult * dl;
+      }
+    }
     while (R) {
       const int q = __ffsll((long long)R) - 1;
       R &= R - 1;
