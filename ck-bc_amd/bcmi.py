@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbcmi.so")
+# BCMI_LIB names another build of the library (A/B comparisons of build switches)
+LIB_PATH = os.environ.get("BCMI_LIB") or os.path.join(HERE, "libbcmi.so")
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_OVERFLOW, ERR_STATE, ERR_IO, ERR_RANGE, ERR_BUSY = range(1, 9)

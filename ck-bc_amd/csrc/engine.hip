@@ -568,7 +568,9 @@ __device__ __forceinline__ void block_add_uniform(long long v, u64* dst) {
 // 64 / PLAIN_PK to a wave iteration, one sub-group of PLAIN_PK lanes each, so
 // that a vertex of a few arcs does not hold a whole wave through its chain
 // of dependent loads.
+#ifndef PLAIN_PK
 #define PLAIN_PK 16
+#endif
 static_assert(64 % PLAIN_PK == 0, "sub-groups tile the wave");
 // The record (header lane) of this lane's sub-group: sub-group k takes the
 // k-th of the next 64 / PLAIN_PK set bits of the wave-uniform mask P, which
