@@ -2310,6 +2310,10 @@ __global__ void k_ms_cleanup(GA g, BA b) {
 // private to their workgroup; within it they are accessed with
 // workgroup-scope atomics (loads served by the L2, which the RMWs update).
 #define WGT 256
+#ifndef WG_ILP
+#define WG_ILP 8  // arcs of a vertex whose loads a workgroup-sweep thread keeps in flight (8: +3 % on
+                  // chains against 4 at 7 waves per SIMD; 16 spills, -9 %)
+#endif
 struct WA {
   const int32_t* src;
   const double* mult;
@@ -2361,7 +2365,14 @@ __device__ __forceinline__ long long block_sum_ll(long long x, long long* red) {
   return t;
 }
 
-__global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
+// WG_WAVES (build switch): waves per SIMD asked of the compiler (2048
+// workgroups of 4 waves fill 256 CUs at 8); unset, the register count decides
+#ifdef WG_WAVES
+#define WG_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(WG_WAVES)))
+#else
+#define WG_WAVES_ATTR
+#endif
+__global__ void __launch_bounds__(WGT) WG_WAVES_ATTR k_wg_sweep(GA g, WA a) {
   const int64_t n = g.n;
   const int tid = threadIdx.x;
   const int64_t slot = blockIdx.x;
@@ -2409,15 +2420,15 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
         const u64 sv = wg_ld(sig + v);
         const int64_t e0 = g.off[v], e1 = g.off[v + 1];
         m_cnt += e1 - e0;
-        // arcs four at a time: their neighbour ids and depths load together
-        for (int64_t eb = e0; eb < e1; eb += 4) {
-          int32_t ws[4], ds[4];
+        // arcs WG_ILP at a time: their neighbour ids and depths load together
+        for (int64_t eb = e0; eb < e1; eb += WG_ILP) {
+          int32_t ws[WG_ILP], ds[WG_ILP];
 #pragma unroll
-          for (int t = 0; t < 4; t++) ws[t] = eb + t < e1 ? g.adj[eb + t] : -1;
+          for (int t = 0; t < WG_ILP; t++) ws[t] = eb + t < e1 ? g.adj[eb + t] : -1;
 #pragma unroll
-          for (int t = 0; t < 4; t++) ds[t] = ws[t] >= 0 ? wg_ld(depth + ws[t]) : INT32_MAX;
+          for (int t = 0; t < WG_ILP; t++) ds[t] = ws[t] >= 0 ? wg_ld(depth + ws[t]) : INT32_MAX;
 #pragma unroll
-          for (int t = 0; t < 4; t++) {
+          for (int t = 0; t < WG_ILP; t++) {
             if (ws[t] < 0) continue;
             const int32_t w = ws[t];
             int32_t dw = ds[t];
@@ -2481,19 +2492,19 @@ __global__ void __launch_bounds__(WGT) k_wg_sweep(GA g, WA a) {
           const double sv = sc ? svr * inv_m : svr;
           double acc = g.w ? (double)g.w[v] : 0.0;
           const int64_t f0 = g.off[v], f1 = g.off[v + 1];
-          for (int64_t eb = f0; eb < f1; eb += 4) {
-            // four successors' loads in flight, the sum still in adjacency order
-            int32_t zs[4];
-            bool ok[4];
-            double zc[4];
+          for (int64_t eb = f0; eb < f1; eb += WG_ILP) {
+            // WG_ILP successors' loads in flight, the sum still in adjacency order
+            int32_t zs[WG_ILP];
+            bool ok[WG_ILP];
+            double zc[WG_ILP];
 #pragma unroll
-            for (int t = 0; t < 4; t++) zs[t] = eb + t < f1 ? g.adj[eb + t] : -1;
+            for (int t = 0; t < WG_ILP; t++) zs[t] = eb + t < f1 ? g.adj[eb + t] : -1;
 #pragma unroll
-            for (int t = 0; t < 4; t++) ok[t] = zs[t] >= 0 && wg_ld(depth + zs[t]) == d + 1;
+            for (int t = 0; t < WG_ILP; t++) ok[t] = zs[t] >= 0 && wg_ld(depth + zs[t]) == d + 1;
 #pragma unroll
-            for (int t = 0; t < 4; t++) zc[t] = ok[t] ? ub2d(wg_ld(sig + zs[t])) : 0.0;
+            for (int t = 0; t < WG_ILP; t++) zc[t] = ok[t] ? ub2d(wg_ld(sig + zs[t])) : 0.0;
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int t = 0; t < WG_ILP; t++)
               if (ok[t]) acc = acc + zc[t] * sv;
           }
           if (acc != 0.0) atomicAdd(a.bc + v, mult * acc);
