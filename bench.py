@@ -373,7 +373,7 @@ def main():
                    "n": g.n, "arcs": g.m, "batch": width,
                    "tuned": {k: v for k, v in (tuned or {}).items() if k in
                              ("chunk", "block", "grid", "grid_back", "alpha_batched", "beta_batched", "gamma_push", "fuse_bu", "pack_bu",
-                              "gamma_back", "steps", "width", "narrow_sigma")}},
+                              "gamma_back", "steps", "width", "narrow_sigma", "concurrency")}},
         "plain_gteps": round(gteps_plain, 3) if gteps_plain else None,
         "batched_over_plain": round(gteps / gteps_plain, 2) if gteps_plain else None,
         "model_bytes_gbs": round(st["bytes"] / t_batched / 1e9, 1),

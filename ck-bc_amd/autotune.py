@@ -5,8 +5,9 @@ space: record size (edges per work item, which needs a fresh upload), block
 shape, forward and backward grids, the top-down -> bottom-up switch
 threshold and the bottom-up floor, the frontier size below which path counts
 are pushed, bottom-up fusion and packing, pushed backward levels, the levels
-per captured graph, the batch width and the path-count row width; then the plain sweep's switch
-threshold and grid.  (The small-grid size for sparse levels is left at the
+per captured graph, the batch width, the path-count row width and the
+batches run at once (each on its own workspace and stream; 0 is the engine's
+choice); then the plain sweep's switch threshold and grid.  (The small-grid size for sparse levels is left at the
 engine default, which sweeps on bridged-block chains set; R-MAT graphs do not
 depend on it.)
 
@@ -49,15 +50,16 @@ SPACE = {
     "steps": [6, 8, 10, 12, 16],
     "width": [32, 48, 64],
     "narrow_sigma": [1, 2],
+    "concurrency": [0, 1, 2, 3, 4],
 }
 PLAIN_SPACE = {"alpha_plain": [100, 300, 1000, 3000, 10000, 1 << 60], "grid_plain": [1024, 2048, 4096, 8192]}
 DEFAULTS = {"chunk": 256, "block": 256, "grid": 4096, "grid_back": 0, "alpha_batched": 1000, "beta_batched": 16,
             "gamma_push": 8, "fuse_bu": 2, "pack_bu": 1, "gamma_back": 0, "steps": 16, "width": 64,
-            "narrow_sigma": 1, "alpha_plain": 1000, "grid_plain": 4096}
+            "narrow_sigma": 1, "concurrency": 0, "alpha_plain": 1000, "grid_plain": 4096}
 DEFAULT_PATH = os.path.join(HERE, "tuned.json")
 # engine parameters a table entry may set (the rest of an entry are notes)
 APPLIED = ("chunk", "block", "grid", "grid_back", "grid_plain", "alpha_batched", "beta_batched", "gamma_push",
-           "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width", "narrow_sigma")
+           "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width", "narrow_sigma", "concurrency")
 NOTES = ("batched_gteps", "plain_gteps", "tuned_on", "tuned_by", "command", "sample")
 
 

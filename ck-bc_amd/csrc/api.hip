@@ -124,6 +124,9 @@ static int param_set(Params& p, const std::string& k, int64_t v, bool have_graph
   } else if (k == "wg_slots") {
     if (v < 1 || v > 65536) return set_error(BCMI_ERR_ARG, "wg_slots must be in [1,65536]");
     p.wg_slots = (int)v;
+  } else if (k == "concurrency") {
+    if (v < 0 || v > 16) return set_error(BCMI_ERR_ARG, "concurrency must be in [0,16]");
+    p.concurrency = (int)v;
   } else if (k == "deg2_slots") {
     if (v < 0 || v > (1 << 20)) return set_error(BCMI_ERR_ARG, "deg2_slots must be in [0,2^20]");
     p.deg2_slots = (int)v;
@@ -265,6 +268,7 @@ int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
   else if (k == "wg_mem") *v = p.wg_mem;
   else if (k == "wg_slots") *v = p.wg_slots;
   else if (k == "deg2_slots") *v = p.deg2_slots;
+  else if (k == "concurrency") *v = p.concurrency;
   else if (k == "deg2_mem") *v = p.deg2_mem;
   else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
   return BCMI_OK;

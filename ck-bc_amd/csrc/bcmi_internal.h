@@ -45,6 +45,7 @@ struct Params {
   int sgrid = 512;          // blocks per kernel in the small-grid (sparse-level) graphs
   int64_t sparse_verts = 16384; // levels below this many vertices replay the small-grid graphs
   int streams = 4;          // sweeps of one call run at once on deep graphs (own workspace and stream each)
+  int concurrency = 0;      // sweeps of one call run at once on any graph (0: `streams` on deep graphs, else 1)
   int strategy = 0;         // 0 auto, 1 device-wide level loop, 2 one workgroup per source
   int wg_depth = 128;       // auto: graphs whose hub BFS depth reaches this use a workgroup per source
   int64_t wg_mem = 16ll << 30;  // device bytes for the per-source state of the workgroup sweeps

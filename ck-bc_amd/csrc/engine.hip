@@ -3233,11 +3233,15 @@ static int64_t record_capacity(const DevGraph& g, int mode) {
   return mode == BCMI_MODE_PLAIN ? base : mult * base;
 }
 
-// Sweeps of one call that run at once on deep graphs (hub BFS depth >= 64):
-// their levels are small and latency-bound, so several sweeps on their own
-// streams overlap well.  Shallow graphs keep the device busy with one.
+// Sweeps of one call that run at once, each on its own workspace and
+// stream.  Deep graphs (hub BFS depth >= 64) have small, latency-bound levels,
+// so several sweeps overlap well (`streams`).  On shallow graphs a second
+// sweep fills what one leaves idle (its small first and last levels, the
+// kernel boundaries, the tail of each level): R-MAT 22 batches 1,245 -> 1,367
+// GTEPS with two, R-MAT 18 + trees 733 -> 956.
 static int concurrent_sweeps(const DevGraph& g, const Params& prm) {
-  return g.depth_hint >= 64 ? std::max(1, std::min(prm.streams, 16)) : 1;
+  if (prm.concurrency > 0) return std::min(prm.concurrency, 16);
+  return g.depth_hint >= 64 ? std::max(1, std::min(prm.streams, 16)) : 2;
 }
 
 static bool use_wg(const DevGraph& g, const Params& prm);
@@ -3880,9 +3884,22 @@ static int run_plain_derived(const DevGraph& g, Workspace* w, const Params& prm,
 // whose other anchor is in a slot runs its derived sources.  A full set of
 // slots gives up the anchor with fewest pairs left; a derived
 // source whose anchor left the slots before its partner ran is swept
-// normally, as is every other listed source.
-static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, const std::vector<int32_t>& cs,
-                          const std::vector<double>& cm, RunPart& P) {
+// normally.  Other listed sources are not the plan's: they run as usual, on
+// the other workspaces of the call at the same time.
+struct Deg2Pair {
+  int32_t a, b;
+  std::vector<size_t> items;  // indices into the source list of the derived sources
+  bool done = false;
+};
+struct Deg2Plan {
+  std::vector<Deg2Pair> pairs;
+  std::unordered_map<int32_t, std::vector<int>> pairs_at;  // anchor -> its pairs
+  std::unordered_map<int32_t, size_t> anchor_item;         // anchor -> its first listed occurrence
+  std::vector<int32_t> order;                              // anchors in run order
+  std::vector<uint8_t> mine;                               // source-list entries the plan runs
+};
+
+static int deg2_plan(const DevGraph& g, Workspace* w, const std::vector<int32_t>& cs, Deg2Plan& D) {
   const int64_t n = g.n;
   const std::vector<int64_t>& ho = g.h_off;
   hipStream_t st = w->st;
@@ -3916,12 +3933,7 @@ static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, co
   }
   // roles: 1 anchor, 2 derived; pairs of anchors with the derived vertices they serve
   std::vector<uint8_t> role(n, 0);
-  struct Pair {
-    int32_t a, b;
-    std::vector<size_t> items;  // indices into cs of the derived sources
-    bool done = false;
-  };
-  std::vector<Pair> pairs;
+  std::vector<Deg2Pair>& pairs = D.pairs;
   std::map<std::pair<int32_t, int32_t>, int> pid;
   std::unordered_map<int32_t, int> pair_of;  // derived vertex -> pair
   for (size_t i = 0; i < cand.size(); i++) {
@@ -3933,16 +3945,22 @@ static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, co
     auto it = pid.find(key);
     if (it == pid.end()) {
       it = pid.emplace(key, (int)pairs.size()).first;
-      pairs.push_back(Pair{key.first, key.second, {}, false});
+      pairs.push_back(Deg2Pair{key.first, key.second, {}, false});
     }
     pair_of[v0] = it->second;
   }
-  std::unordered_map<int32_t, std::vector<int>> pairs_at;   // anchor -> its pairs
-  std::unordered_map<int32_t, size_t> anchor_item;          // anchor -> its first listed occurrence
+  std::unordered_map<int32_t, std::vector<int>>& pairs_at = D.pairs_at;
+  std::unordered_map<int32_t, size_t>& anchor_item = D.anchor_item;
+  D.mine.assign(cs.size(), 0);
   for (size_t i = 0; i < cs.size(); i++) {
     const int32_t v = cs[i];
-    if (role[v] == 2) pairs[pair_of[v]].items.push_back(i);
-    else if (role[v] == 1 && !anchor_item.count(v)) anchor_item[v] = i;
+    if (role[v] == 2) {
+      pairs[pair_of[v]].items.push_back(i);
+      D.mine[i] = 1;
+    } else if (role[v] == 1 && !anchor_item.count(v)) {
+      anchor_item[v] = i;
+      D.mine[i] = 1;
+    }
   }
   for (size_t q = 0; q < pairs.size(); q++) {
     pairs_at[pairs[q].a].push_back((int)q);
@@ -3955,7 +3973,7 @@ static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, co
     size_t px = pairs_at[x].size(), py = pairs_at[y].size();
     return px != py ? px > py : x < y;
   });
-  std::vector<int32_t> order;
+  std::vector<int32_t>& order = D.order;
   {
     std::unordered_map<int32_t, char> seen;
     for (int32_t r : anchors) {
@@ -3979,6 +3997,20 @@ static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, co
       }
     }
   }
+  return BCMI_OK;
+}
+
+// Runs a plan's anchors and derived sources on workspace w (the one whose
+// slots hold the anchors' states); derived sources whose pair did not
+// complete are swept normally here.
+static int deg2_run(const DevGraph& g, Workspace* w, const Params& prm, const std::vector<int32_t>& cs,
+                    const std::vector<double>& cm, Deg2Plan& D, RunPart& P) {
+  const int64_t n = g.n;
+  hipStream_t st = w->st;
+  std::vector<Deg2Pair>& pairs = D.pairs;
+  std::unordered_map<int32_t, std::vector<int>>& pairs_at = D.pairs_at;
+  std::unordered_map<int32_t, size_t>& anchor_item = D.anchor_item;
+  const std::vector<int32_t>& order = D.order;
   std::vector<uint8_t> ran(cs.size(), 0);
   int S = (int)std::min<int64_t>(prm.deg2_slots > 0 ? prm.deg2_slots
                                                     : std::max<int64_t>(2, prm.deg2_mem / (12 * std::max<int64_t>(1, n))),
@@ -4034,7 +4066,7 @@ static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, co
       TRY(sweep_cleanup(g, w, prm));
       if (!usable) continue;
       for (int pq : pairs_at[a]) {
-        Pair& pr = pairs[pq];
+        Deg2Pair& pr = pairs[pq];
         const int32_t b = pr.a == a ? pr.b : pr.a;
         if (pr.done || !slot_of.count(b)) continue;
         const int sa = slot_of[a], sb = slot_of[b];
@@ -4052,7 +4084,7 @@ static int run_plain_deg2(const DevGraph& g, Workspace* w, const Params& prm, co
     }
   }
   for (size_t i = 0; i < cs.size(); i++)
-    if (!ran[i]) TRY(run_plain_one(g, w, prm, cs[i], cm[i], P));
+    if (D.mine[i] && !ran[i]) TRY(run_plain_one(g, w, prm, cs[i], cm[i], P));
   return BCMI_OK;
 }
 
@@ -4399,11 +4431,30 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
     cs.swap(cs2);
     cm.swap(cm2);
   }
-  const size_t nitems = wg ? 0 : plain ? cs.size() : (cs.size() + W - 1) / W;
-  const int K = (int)std::min<size_t>((size_t)concurrent_sweeps(g, prm), std::max<size_t>(1, nitems));
-  if (K > 1) TRY(ensure_helpers(g, ws, prm, K - 1));
-  // 2-degree sources: plain sweeps on one workspace of an undirected graph
-  const bool deg2 = (src_flags & BCMI_SRC_DEG2) && plain && !wg && K == 1 && !g.directed && prm.deg2_slots != 1;
+  // 2-degree heuristic: the plan's anchors and derived sources run on this
+  // workspace (whose slots hold the anchors' states), the other sources on
+  // every workspace of the call
+  const bool deg2 = (src_flags & BCMI_SRC_DEG2) && plain && !wg && !g.directed && prm.deg2_slots != 1;
+  Deg2Plan plan;
+  std::vector<size_t> items;  // work items of the shared queue: sources (plain) or batches
+  if (deg2) {
+    TRY(deg2_plan(g, ws, cs, plan));
+    for (size_t i = 0; i < cs.size(); i++)
+      if (!plan.mine[i]) items.push_back(i);
+  } else if (!wg) {
+    items.resize(plain ? cs.size() : (cs.size() + W - 1) / W);
+    for (size_t i = 0; i < items.size(); i++) items[i] = i;
+  }
+  const size_t nitems = items.size() + (deg2 ? 1 : 0);
+  int K = (int)std::min<size_t>((size_t)concurrent_sweeps(g, prm), std::max<size_t>(1, nitems));
+  if (K > 1) {
+    // as many helper workspaces as the device holds (fewer sweeps at once otherwise)
+    const int rc = ensure_helpers(g, ws, prm, K - 1);
+    if (rc != BCMI_OK) {
+      if (rc != BCMI_ERR_NOMEM || ws->helpers_graph != &g) return rc;
+      K = 1 + (int)ws->helpers.size();
+    }
+  }
   std::vector<RunPart> parts(K);
   std::atomic<size_t> next{0};
   std::atomic<bool> stop{false};
@@ -4418,18 +4469,19 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
       return;
     }
     Workspace* w = t == 0 ? ws : ws->helpers[t - 1];
-    if (deg2) {
-      // one workspace (K == 1): the derived sources need their neighbours' saved states
-      int rc = run_plain_deg2(g, w, prm, cs, cm, P);
+    if (deg2 && t == 0) {
+      const int rc = deg2_run(g, w, prm, cs, cm, plan, P);
       if (rc != BCMI_OK) {
         P.rc = rc;
         P.msg = bcmi_last_error();
+        stop = true;
+        return;
       }
-      return;
     }
     while (!stop) {
-      size_t it = next.fetch_add(1);
-      if (it >= nitems) break;
+      const size_t q = next.fetch_add(1);
+      if (q >= items.size()) break;
+      const size_t it = items[q];
       int rc;
       if (plain) {
         rc = run_plain_one(g, w, prm, cs[it], cm[it], P);

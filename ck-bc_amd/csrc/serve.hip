@@ -242,6 +242,9 @@ int bcmi_server_create(int device, int64_t hbm_budget, int max_slots, bcmi_serve
   s->budget = hbm_budget;
   s->max_slots = max_slots;
   s->t0 = now_ms();
+  // queries already overlap on the slots: each runs its batches one at a time
+  // (no helper workspaces outside the budget's accounting of one per slot)
+  s->prm.concurrency = 1;
   s->sched = std::thread(scheduler, s);
   for (int i = 0; i < max_slots; i++) s->workers.emplace_back(worker, s);
   *out = s;

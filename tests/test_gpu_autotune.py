@@ -1,5 +1,6 @@
 """The autotuner end to end on a small graph: tune() with a reduced space on
-R-MAT scale 12 returns parameters from that space; an engine with them
+R-MAT scale 12 returns parameters from that space or the defaults it starts
+from; an engine with them
 applied (autotune.apply, as bench.py and the CK run commands do) holds them
 and matches the CPU oracle."""
 import numpy as np
@@ -20,9 +21,11 @@ def test_tune_reduced_space_then_oracle():
                                      plain_space={"grid_plain": [256, 1024]}, passes=1, log=lambda m: None)
     assert score > 0 and pscore > 0
     assert set(p) == set(autotune.APPLIED)
+    # a coordinate keeps its starting value (the default) unless a value of the
+    # space beats it by more than the minimum gain
     for k, vals in space.items():
-        assert p[k] in vals, k
-    assert p["grid_plain"] in (256, 1024)
+        assert p[k] in vals or p[k] == autotune.DEFAULTS[k], k
+    assert p["grid_plain"] in (256, 1024, autotune.DEFAULTS["grid_plain"])
     for k in set(autotune.APPLIED) - set(space) - {"grid_plain"}:
         assert p[k] == autotune.DEFAULTS[k], k
     key = autotune.graph_key("rmat", scale=12, edgefactor=16)

@@ -206,7 +206,7 @@ def test_tuned_table_names_known_parameters():
     assert {"default", "rmat-s18-ef16", "rmat-s22-ef16", "rmat-s25-ef16"} <= set(table)
     notes = set(autotune.NOTES)
     applied = {"chunk", "block", "grid", "grid_back", "grid_plain", "alpha_batched", "beta_batched", "gamma_push",
-               "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width", "narrow_sigma"}
+               "fuse_bu", "pack_bu", "gamma_back", "steps", "alpha_plain", "width", "narrow_sigma", "concurrency"}
     assert set(autotune.APPLIED) == applied
 
     class Rec:
@@ -220,6 +220,7 @@ def test_tuned_table_names_known_parameters():
         assert set(entry) <= applied | notes, (key, set(entry) - applied - notes)
         assert entry["block"] in (64, 128, 256) and entry["chunk"] >= 64 and 1 <= entry["width"] <= 64
         assert entry["steps"] >= 1 and entry["grid"] >= 1 and entry.get("narrow_sigma", 1) in (0, 1, 2)
+        assert 0 <= entry.get("concurrency", 0) <= 16
         r = Rec()
         autotune.apply(r, table, key)
         assert r.set == {k: int(v) for k, v in entry.items() if k in applied}
