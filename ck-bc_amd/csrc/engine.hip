@@ -4454,6 +4454,11 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
       if (rc != BCMI_ERR_NOMEM || ws->helpers_graph != &g) return rc;
       K = 1 + (int)ws->helpers.size();
     }
+    // the kernel timer sums the timed kernel's launches on every workspace
+    for (int t = 1; t < K; t++) {
+      Workspace* h = ws->helpers[t - 1];
+      hipLaunchKernelGGL(k_kt_reset, 1, 1, 0, h->st, h->kt, h->kt_ws, h->kt_we);
+    }
   }
   std::vector<RunPart> parts(K);
   std::atomic<size_t> next{0};
@@ -4539,13 +4544,22 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   HIP_TRY(hipEventElapsedTime(&ms, ws->ev0, ws->ev1));
   R.gpu_ms = ms;
   R.wall_ms = now_ms() - t0;
-  HIP_TRY(hipMemcpyAsync(ws->hkt, ws->kt, sizeof(KT), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
   {
+    // timed kernel: launch durations and launches summed over the workspaces
+    // that ran sweeps (launches of concurrent sweeps overlap, so each one's
+    // duration includes the device time the others took meanwhile)
+    u64 total = 0, launches = 0;
+    for (int t = 0; t < K; t++) {
+      Workspace* w = t == 0 ? ws : ws->helpers[t - 1];
+      HIP_TRY(hipMemcpyAsync(w->hkt, w->kt, sizeof(KT), hipMemcpyDeviceToHost, w->st));
+      HIP_TRY(hipStreamSynchronize(w->st));
+      total += w->hkt->total;
+      launches += w->hkt->launches;
+    }
     int khz = 0;
     HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    R.ktime_ms = khz > 0 ? (double)ws->hkt->total / (double)khz : 0.0;
-    R.klaunches = (int64_t)ws->hkt->launches;
+    R.ktime_ms = khz > 0 ? (double)total / (double)khz : 0.0;
+    R.klaunches = (int64_t)launches;
   }
   for (size_t i = 0; i < R.n_s.size(); i++) {
     R.sum_n += R.n_s[i]; R.sum_m += R.m_s[i]; R.sum_a += R.a_s[i];
