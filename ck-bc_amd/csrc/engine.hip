@@ -2311,8 +2311,8 @@ __global__ void k_ms_cleanup(GA g, BA b) {
 // workgroup-scope atomics (loads served by the L2, which the RMWs update).
 #define WGT 256
 #ifndef WG_ILP
-#define WG_ILP 8  // arcs of a vertex whose loads a workgroup-sweep thread keeps in flight (8: +3 % on
-                  // chains against 4 at 7 waves per SIMD; 16 spills, -9 %)
+#define WG_ILP 4  // arcs of a vertex whose loads (then claims, then adds) a workgroup-sweep thread keeps
+                  // in flight (4 at 8 waves per SIMD: +6 % on chains against 8 claims one at a time)
 #endif
 struct WA {
   const int32_t* src;
@@ -2365,13 +2365,12 @@ __device__ __forceinline__ long long block_sum_ll(long long x, long long* red) {
   return t;
 }
 
-// WG_WAVES (build switch): waves per SIMD asked of the compiler (2048
-// workgroups of 4 waves fill 256 CUs at 8); unset, the register count decides
-#ifdef WG_WAVES
-#define WG_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(WG_WAVES)))
-#else
-#define WG_WAVES_ATTR
+// waves per SIMD asked of the compiler: 2048 workgroups of 4 waves fill 256
+// CUs at 8 (build switch WG_WAVES)
+#ifndef WG_WAVES
+#define WG_WAVES 8
 #endif
+#define WG_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(WG_WAVES)))
 __global__ void __launch_bounds__(WGT) WG_WAVES_ATTR k_wg_sweep(GA g, WA a) {
   const int64_t n = g.n;
   const int tid = threadIdx.x;
@@ -2427,31 +2426,43 @@ __global__ void __launch_bounds__(WGT) WG_WAVES_ATTR k_wg_sweep(GA g, WA a) {
           for (int t = 0; t < WG_ILP; t++) ws[t] = eb + t < e1 ? g.adj[eb + t] : -1;
 #pragma unroll
           for (int t = 0; t < WG_ILP; t++) ds[t] = ws[t] >= 0 ? wg_ld(depth + ws[t]) : INT32_MAX;
+          // The claims of unvisited neighbours are all issued before any
+          // result is used, then the path-count adds likewise: two atomic
+          // round trips per WG_ILP arcs instead of up to two per arc.
+          int32_t dw[WG_ILP];
 #pragma unroll
           for (int t = 0; t < WG_ILP; t++) {
-            if (ws[t] < 0) continue;
-            const int32_t w = ws[t];
-            int32_t dw = ds[t];
-            if (dw < 0) {
-              int32_t expect = -1;
-              if (__hip_atomic_compare_exchange_strong(depth + w, &expect, d1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP)) {
-                const int pos = atomicAdd(&s_tail, 1);
-                wg_st(q + pos, w);
-                dw = d1;
-              } else {
-                dw = expect;
-              }
+            dw[t] = ds[t];
+            if (ws[t] >= 0 && ds[t] < 0) {
+              int32_t e = -1;  // stays -1 when this claim wins, else the claimer's depth
+              __hip_atomic_compare_exchange_strong(depth + ws[t], &e, d1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+              dw[t] = e;
             }
-            if (dw == d1) {
+          }
+#pragma unroll
+          for (int t = 0; t < WG_ILP; t++) {
+            if (ws[t] >= 0 && ds[t] < 0 && dw[t] == -1) {
+              wg_st(q + atomicAdd(&s_tail, 1), ws[t]);
+              dw[t] = d1;
+            }
+          }
+          u64 old[WG_ILP];
+#pragma unroll
+          for (int t = 0; t < WG_ILP; t++) {
+            old[t] = 0;
+            if (ws[t] >= 0 && dw[t] == d1) {
               a_cnt++;
-              if (sc) {
-                __hip_atomic_fetch_add((double*)(sig + w), ub2d(sv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              } else {
-                u64 old = __hip_atomic_fetch_add(sig + w, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (old + sv < old) s_flag = 1;
-              }
+              if (sc)
+                __hip_atomic_fetch_add((double*)(sig + ws[t]), ub2d(sv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              else
+                old[t] = __hip_atomic_fetch_add(sig + ws[t], sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
+          }
+          if (!sc) {
+#pragma unroll
+            for (int t = 0; t < WG_ILP; t++)
+              if (ws[t] >= 0 && dw[t] == d1 && old[t] + sv < old[t]) s_flag = 1;
           }
         }
       }
