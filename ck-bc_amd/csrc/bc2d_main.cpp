@@ -363,7 +363,9 @@ int main(int argc, char** argv) {
   }
   printf("task 0 BC GTEPS (forward+backward arcs): %lf\n", all_time > 0 ? st.te / all_time / 1e9 : 0.0);
   if (peel) printf("task 0 1-Degree reduction: %lf sec\n", redu_time);
-  if (deg2) printf("task 0 2-Degree sources taken from their neighbours: %" PRId64 "\n", st.deg2_sources);
+  if (deg2)
+    printf("task 0 2-Degree sources (and degree 1 when not peeled) taken from their neighbours: %" PRId64 "\n",
+           st.deg2_sources);
   printf("\n");
   bcmi_destroy(ctx);
   bcmi_graph_free(g);

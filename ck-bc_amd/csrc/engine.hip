@@ -1098,6 +1098,7 @@ struct D2A {
   int32_t v0;
   int64_t* lcnt;      // [level] records of the level, then its write cursor
   int64_t nlev;       // levels lcnt holds: no depth reaches it (min of the two sweeps' levels + 2)
+  int single;         // v0 has degree 1: a and b are its one neighbour (da == db), its side counted once
 };
 
 __global__ void k_d2_init(GA g, PA p, D2A d) {
@@ -1110,7 +1111,7 @@ __global__ void k_d2_init(GA g, PA p, D2A d) {
   c->rec_count = 0;
   c->arcs_next = 0; c->verts_next = 0; c->arcs_cur = 0; c->verts_cur = 0; c->arcs_seen = 0;
   c->arcs_reach = g.comp_arcs[g.label[d.v0]];
-  c->sum_a = 2;  // v0's own arcs, to a and b on level 1
+  c->sum_a = d.single ? 1 : 2;  // v0's own arcs, to its neighbours on level 1
   c->n_td = 0; c->n_bu = 0;
   c->tot_verts = 0; c->tot_arcs = 0; c->uni_verts = 0; c->uni_arcs = 0;
   for (int64_t i = 0; i < d.nlev; i++) d.lcnt[i] = 0;
@@ -1139,7 +1140,7 @@ __global__ void __launch_bounds__(256) k_d2_combine(GA g, PA p, D2A d) {
       const int mn = x < 0 ? y : y < 0 ? x : min(x, y);
       dep = mn + 1;
       sg = x == mn ? d.sa[t] : 0;
-      if (y == mn) {
+      if (y == mn && !d.single) {
         const u64 s2 = sg + d.sb[t];
         if (s2 < sg) ovf = 1;
         sg = s2;
@@ -1232,12 +1233,13 @@ __global__ void k_d2_reset(GA g, PA p, D2A d) {
   }
 }
 
-// The two neighbours of each listed vertex of degree 2.
+// The neighbours of each listed vertex of degree 1 or 2 (degree 1: the one
+// neighbour twice).
 __global__ void k_d2_nb(const int64_t* off, const int32_t* adj, const int32_t* v, int64_t k, int32_t* out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t o = off[v[i]];
+    const int64_t o = off[v[i]], dg = off[v[i] + 1] - o;
     out[2 * i] = adj[o];
-    out[2 * i + 1] = adj[o + 1];
+    out[2 * i + 1] = dg > 1 ? adj[o + 1] : adj[o];
   }
 }
 
@@ -3862,7 +3864,7 @@ static int run_plain_derived(const DevGraph& g, Workspace* w, const Params& prm,
   D2A d;
   d.da = w->d2_depth + (size_t)sa * n; d.sa = w->d2_sigma + (size_t)sa * n;
   d.db = w->d2_depth + (size_t)sb * n; d.sb = w->d2_sigma + (size_t)sb * n;
-  d.v0 = v0; d.lcnt = w->d2_lcnt; d.nlev = nlev;
+  d.v0 = v0; d.lcnt = w->d2_lcnt; d.nlev = nlev; d.single = sa == sb;
   const int G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
   hipLaunchKernelGGL(k_d2_init, 1, 1, 0, st, ga, pa, d);
   hipLaunchKernelGGL(k_d2_combine, G, 256, 0, st, ga, pa, d);
@@ -3920,7 +3922,7 @@ static int deg2_plan(const DevGraph& g, Workspace* w, const std::vector<int32_t>
   auto listed = [&](int32_t v) { return std::binary_search(uniq.begin(), uniq.end(), v); };
   std::vector<int32_t> cand;
   for (int32_t v : uniq)
-    if (ho[v + 1] - ho[v] == 2) cand.push_back(v);
+    if (ho[v + 1] - ho[v] == 2 || ho[v + 1] - ho[v] == 1) cand.push_back(v);
   std::vector<int32_t> nb(2 * cand.size());
   if (!cand.empty()) {
     int32_t *dc = nullptr, *dn = nullptr;
@@ -3949,7 +3951,7 @@ static int deg2_plan(const DevGraph& g, Workspace* w, const std::vector<int32_t>
   std::unordered_map<int32_t, int> pair_of;  // derived vertex -> pair
   for (size_t i = 0; i < cand.size(); i++) {
     const int32_t v0 = cand[i], a = nb[2 * i], b = nb[2 * i + 1];
-    if (role[v0] || a == b || role[a] == 2 || role[b] == 2 || !listed(a) || !listed(b)) continue;
+    if (role[v0] || role[a] == 2 || role[b] == 2 || !listed(a) || !listed(b)) continue;  // (a == b: degree 1)
     role[v0] = 2;
     role[a] = role[b] = 1;
     auto key = std::make_pair(std::min(a, b), std::max(a, b));
@@ -3975,7 +3977,7 @@ static int deg2_plan(const DevGraph& g, Workspace* w, const std::vector<int32_t>
   }
   for (size_t q = 0; q < pairs.size(); q++) {
     pairs_at[pairs[q].a].push_back((int)q);
-    pairs_at[pairs[q].b].push_back((int)q);
+    if (pairs[q].b != pairs[q].a) pairs_at[pairs[q].b].push_back((int)q);
   }
   // anchor order: breadth-first over the pairs, starting from the anchors with most pairs
   std::vector<int32_t> anchors;
@@ -4026,9 +4028,10 @@ static int deg2_run(const DevGraph& g, Workspace* w, const Params& prm, const st
   int S = (int)std::min<int64_t>(prm.deg2_slots > 0 ? prm.deg2_slots
                                                     : std::max<int64_t>(2, prm.deg2_mem / (12 * std::max<int64_t>(1, n))),
                                  (int64_t)order.size());
-  // fewer slots while the device cannot hold them
-  while (S >= 2 && d2_reserve(w, S, 256) != BCMI_OK) S /= 2;
-  if (S >= 2) {
+  // fewer slots while the device cannot hold them (one slot still serves
+  // degree-1 sources, whose group has a single anchor)
+  while (S >= 1 && d2_reserve(w, S, 256) != BCMI_OK) S /= 2;
+  if (S >= 1) {
     std::vector<int32_t> slot_v(S, -1);
     std::vector<int> slot_lv(S, 0);
     std::vector<int64_t> slot_t(S, 0);
@@ -4088,7 +4091,7 @@ static int deg2_run(const DevGraph& g, Workspace* w, const Params& prm, const st
         }
         pr.done = true;
         pending[a]--;
-        pending[b]--;
+        if (b != a) pending[b]--;
         slot_t[sb] = ++clock;
         slot_t[sa] = ++clock;
       }

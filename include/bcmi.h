@@ -52,7 +52,8 @@ extern "C" {
  * c:1015).  OR-ed into src_flags: in BCMI_MODE_PLAIN a listed source of
  * (traversed) degree 2 whose two neighbours are listed too takes its depths
  * and path counts from the saved forward states of their sweeps instead of a
- * forward sweep of its own; BC is unchanged.  Batched runs, directed graphs
+ * forward sweep of its own, and so does a listed source of degree 1 from its
+ * neighbour's (without peeling; peeled cores have none); BC is unchanged.  Batched runs, directed graphs
  * and deep graphs (workgroup sweeps) ignore it. */
 #define BCMI_SRC_DEG2 2
 

@@ -25,6 +25,12 @@ def subdivided(edges, n):
 
 
 def combine(da, sa, db, sb, v0, rule="min"):
+    if db is None:  # degree 1: every path leaves through the one neighbour
+        d0 = np.where(da >= 0, da + 1, -1).astype(np.int32)
+        s0 = np.where(da >= 0, sa, 0).astype(np.uint64)
+        d0[v0] = 0
+        s0[v0] = 1
+        return d0, s0
     ra, rb = da >= 0, db >= 0
     big = np.iinfo(np.int32).max
     xa = np.where(ra, da, big)
@@ -56,6 +62,20 @@ def graphs():
     yield "rand_sub", O.csr_from_edges(*subdivided(*G.random_graph(60, 120, 7)))
     yield "diamonds", O.csr_from_edges(*G.diamonds(9))
     yield "rand", O.csr_from_edges(*G.random_graph(300, 420, 11))
+
+
+@pytest.mark.parametrize("name,g", list(graphs()), ids=lambda x: x if isinstance(x, str) else "")
+def test_degree1_rule_equals_forward_sweep(name, g):
+    """Degree 1: d0(t) = 1 + da(t), sigma0(t) = sigma_a(t) for t != v0."""
+    off, adj = g
+    deg = np.diff(off)
+    leaves = np.nonzero(deg == 1)[0][:100]
+    for v0 in leaves:
+        a = int(adj[off[v0]])
+        d, sg, _, _ = O.sssp(off, adj, int(v0))
+        da, sa, _, _ = O.sssp(off, adj, a)
+        d0, s0 = combine(da, sa, None, None, int(v0))
+        assert np.array_equal(d0, d) and np.array_equal(s0, sg), (name, v0)
 
 
 @pytest.mark.parametrize("name,g", list(graphs()), ids=lambda x: x if isinstance(x, str) else "")

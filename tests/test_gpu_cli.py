@@ -68,7 +68,7 @@ def test_ck_run_two_degree_heuristic(tmp_path, heur):
                         "-P", "-o", str(out)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert "REMOVED" not in r.stdout
-    line = [l for l in r.stdout.splitlines() if "2-Degree sources taken from their neighbours" in l]
+    line = [l for l in r.stdout.splitlines() if "taken from their neighbours" in l]
     assert line and int(line[0].split()[-1]) > 0
     got = read_result(str(out) + "_1X1_0.log")
     want = f["bc_all"]

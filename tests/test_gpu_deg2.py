@@ -108,6 +108,28 @@ def test_deg2_rmat12_all_sources_and_slots(engine):
     assert derived[2] >= 0.9 * np.count_nonzero(deg == 2), derived
 
 
+def test_degree1_sources_from_their_neighbour(engine):
+    """Without peeling, a degree-1 source takes its forward state from its one
+    neighbour's sweep: every leaf of a star and of a tree of paths."""
+    e, n = G.star(9)
+    g = bcmi.Graph.from_edges(e, n)
+    engine.upload(g)
+    S = np.arange(n, dtype=np.int32)
+    bc0, st0, bc1, st1 = runs(engine, S)
+    assert st1["deg2_sources"] == 9          # the leaves, from the centre's sweep
+    assert bcmi.rel_err(bc1, O.brandes(g.off, g.adj, S)) <= BC_TOL
+    same_totals(st0, st1)
+    e, n = G.random_forest_plus(200, 7)
+    g = bcmi.Graph.from_edges(e, n)
+    engine.upload(g)
+    S = np.arange(n, dtype=np.int32)
+    bc0, st0, bc1, st1 = runs(engine, S)
+    deg = np.diff(g.off)
+    assert st1["deg2_sources"] >= np.count_nonzero(deg == 1) // 3
+    assert bcmi.rel_err(bc1, O.brandes(g.off, g.adj, S, nthreads=4)) <= BC_TOL
+    same_totals(st0, st1)
+
+
 def test_deg2_duplicates_and_subsets(engine):
     e, n = subdivided(*G.grid(7, 5))
     g = bcmi.Graph.from_edges(e, n)
