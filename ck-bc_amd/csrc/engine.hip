@@ -159,6 +159,15 @@ __device__ __forceinline__ u64 ld_coherent_u64(const u64* p) {
 #ifndef UNR
 #define UNR 8      // row gathers in flight per wave in the sigma and backward loops
 #endif
+#ifndef UNR_BACK
+#define UNR_BACK UNR  // row gathers in flight per wave in the backward loop
+#endif
+#ifndef BU_HDR_LDS
+#define BU_HDR_LDS 1  // bottom-up record headers held in LDS instead of registers
+#endif
+#ifndef BACK_HDR_LDS
+#define BACK_HDR_LDS 1  // backward record headers held in LDS instead of registers
+#endif
 #define KF_GATE 1  // gate mask probes with the level bitmaps
 #define KF_NT 2    // non-temporal adjacency loads
 // One bit per vertex: "this vertex has a non-zero mask on this level".
@@ -1420,7 +1429,7 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 #define BU_PIPE 1  // bottom-up slabs' adjacency and mask loads run ahead of the gathers
 #endif
 #ifndef BU_WAVES
-#define BU_WAVES 5
+#define BU_WAVES 6
 #endif
 #ifndef UNR_BU
 #define UNR_BU 12  // parent gathers in flight per wave in the bottom-up kernel
@@ -1429,8 +1438,23 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 #define PACK_DEG 16  // packed bottom-up: vertices of degree <= PACK_DEG ...
 #define PACK_W 4     // ... PACK_W to a wave (PACK_DEG * PACK_W = 64)
 static_assert(PACK_DEG * PACK_W == 64 && PACK_W == 4, "the packed bottom-up scan picks one vertex per sub-group of 16 lanes");
+// Headers of 64 records for each of a block's four waves, held in LDS while
+// the wave runs the records (BACK_HDR_LDS, BU_HDR_LDS): vertex, record index,
+// a mask, first and end arc.
+struct RecHdr {
+  int32_t v[MAX_WAVES_PER_BLOCK][64], j[MAX_WAVES_PER_BLOCK][64];
+  u64 L[MAX_WAVES_PER_BLOCK][64];
+  int64_t o0[MAX_WAVES_PER_BLOCK][64], o1[MAX_WAVES_PER_BLOCK][64];
+};
+// LDS of the bottom-up kernels, declared once per kernel (not once per
+// template instance): the appender's staging and the record headers.
+struct BuLds {
+  int32_t app_lv[MAX_WAVES_PER_BLOCK][WBUF];
+  int32_t app_lj[MAX_WAVES_PER_BLOCK][WBUF];
+  RecHdr hdr;
+};
 template <int NW, bool PACKED>
-__device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
+__device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
   Ctl* c = b.ctl;
   if (c->fdone || c->mode != 1) return;
   const int d = c->level;
@@ -1447,7 +1471,8 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
   u64 ca = 0;
   int ovf = 0;
   double mx = 0.0;
-  APPENDER(ap);
+  Appender ap;
+  app_init(ap, sh->app_lv, sh->app_lj);
   const bool pk = b.pack;
   const int32_t* rv = pk ? g.lg_v : g.all_v;
   const int32_t* rjv = pk ? g.lg_j : g.all_j;
@@ -1472,13 +1497,28 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       open = svq != b.compmask[g.label[vq]];
     }
     u64 O = __ballot(open);
+#if BU_HDR_LDS
+    // the headers wait in LDS (a wave reads back only its own slots, and a
+    // wave's LDS accesses complete in issue order)
+    const int wb = threadIdx.x >> 6;
+    sh->hdr.v[wb][lane] = vq; sh->hdr.j[wb][lane] = jq; sh->hdr.L[wb][lane] = svq;
+    sh->hdr.o0[wb][lane] = o0q; sh->hdr.o1[wb][lane] = o1q;
+    __builtin_amdgcn_wave_barrier();
+#endif
     while (O) {
       const int oq = __ffsll((long long)O) - 1;
       O &= O - 1;
+#if BU_HDR_LDS
+      const int32_t v = __builtin_amdgcn_readfirstlane(sh->hdr.v[wb][oq]);
+      const u64 sv = rl64(sh->hdr.L[wb][oq], 0);
+      const int j = __builtin_amdgcn_readfirstlane(sh->hdr.j[wb][oq]);
+      const int64_t o0 = (int64_t)rl64((u64)sh->hdr.o0[wb][oq], 0), o1 = (int64_t)rl64((u64)sh->hdr.o1[wb][oq], 0);
+#else
       const int32_t v = rl32(vq, oq);
       const u64 sv = rl64(svq, oq);
       const int j = rl32(jq, oq);
       const int64_t o0 = (int64_t)rl64((u64)o0q, oq), o1 = (int64_t)rl64((u64)o1q, oq);
+#endif
       const int64_t deg = o1 - o0;
       int64_t e0 = o0 + (int64_t)j * g.chunk;
       int64_t e1 = min(o1, e0 + g.chunk);
@@ -1589,6 +1629,9 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
       }
       app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
     }
+#if BU_HDR_LDS
+    __builtin_amdgcn_wave_barrier();
+#endif
   }
   if (PACKED) {
     // Vertices of degree <= PACK_DEG, PACK_W per wave: sub-group q (lanes
@@ -1724,10 +1767,11 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b) {
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BU_WAVES))) k_ms_bu(GA g, BA b, KT* kt) {
   kt_enter(kt);
+  __shared__ BuLds sh;
   const int nw = b.ctl->narrow;
-  if (nw == 2) k_ms_bu_body<2, false>(g, b);
-  else if (nw == 1) k_ms_bu_body<1, false>(g, b);
-  else k_ms_bu_body<0, false>(g, b);
+  if (nw == 2) k_ms_bu_body<2, false>(g, b, &sh);
+  else if (nw == 1) k_ms_bu_body<1, false>(g, b, &sh);
+  else k_ms_bu_body<0, false>(g, b, &sh);
   kt_exit(kt);
 }
 
@@ -1735,10 +1779,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BU_WAV
 // 5 waves per SIMD, 96 VGPRs; its load chains need the extra resident wave).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ms_bu_pk(GA g, BA b, KT* kt) {
   kt_enter(kt);
+  __shared__ BuLds sh;
   const int nw = b.ctl->narrow;
-  if (nw == 2) k_ms_bu_body<2, true>(g, b);
-  else if (nw == 1) k_ms_bu_body<1, true>(g, b);
-  else k_ms_bu_body<0, true>(g, b);
+  if (nw == 2) k_ms_bu_body<2, true>(g, b, &sh);
+  else if (nw == 1) k_ms_bu_body<1, true>(g, b, &sh);
+  else k_ms_bu_body<0, true>(g, b, &sh);
   kt_exit(kt);
 }
 
@@ -2034,10 +2079,10 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
     }
     // Successors UNR at a time (loads first, then the sums in adjacency
     // order; FP contraction is off, so each step rounds like the oracle).
-    for_lanes<UNR>(__ballot(M != 0), [&](const int (&q)[UNR], const bool (&ok)[UNR]) {
-      double x[UNR];
+    for_lanes<UNR_BACK>(__ballot(M != 0), [&](const int (&q)[UNR_BACK], const bool (&ok)[UNR_BACK]) {
+      double x[UNR_BACK];
 #pragma unroll
-      for (int t = 0; t < UNR; t++) {
+      for (int t = 0; t < UNR_BACK; t++) {
         const int32_t zq = rl32(z, q[t]);
         const u64 mq = ok[t] ? rl64(M, q[t]) : 0ull;
         x[t] = 0.0;
@@ -2046,7 +2091,7 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
       // a lane outside a successor's mask adds 0 * sv = +0 (sv is positive
       // and finite, acc never -0), which leaves acc unchanged
 #pragma unroll
-      for (int t = 0; t < UNR; t++) acc = acc + x[t] * sv;
+      for (int t = 0; t < UNR_BACK; t++) acc = acc + x[t] * sv;
     });
   }
   double* crow = b.coef + (int64_t)v * 64 + lane;
@@ -2069,7 +2114,7 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
 }
 
 template <int NW>
-__device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
+__device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
   Ctl* c = b.ctl;
   if (!c->fdone || c->bdone) return;
   const int d = c->blevel;
@@ -2101,6 +2146,17 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
   } else {
     // This wave's records lo + gw, lo + gw + tw, ...: the headers of 64 of
     // them load at once, one lane each, then the records run in order.
+#if BACK_HDR_LDS
+    // the headers wait in LDS (a wave reads back only its own slots, and a
+    // wave's LDS accesses complete in issue order), which leaves their
+    // registers to the row gathers
+    int32_t (&h_v)[4][64] = hdr->v;
+    int32_t (&h_j)[4][64] = hdr->j;
+    u64 (&h_L)[4][64] = hdr->L;
+    int64_t (&h_o0)[4][64] = hdr->o0;
+    int64_t (&h_o1)[4][64] = hdr->o1;
+    const int wb = threadIdx.x >> 6;
+#endif
     for (int64_t k0 = 0; lo + gw + k0 * tw < hi; k0 += 64) {
       const int64_t rq = lo + gw + (k0 + lane) * tw;
       int32_t vq = 0, jq = 0;
@@ -2114,12 +2170,27 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
         o1q = g.off[vq + 1];
       }
       u64 R = __ballot(rq < hi);
+#if BACK_HDR_LDS
+      h_v[wb][lane] = vq; h_j[wb][lane] = jq; h_L[wb][lane] = Lq; h_o0[wb][lane] = o0q; h_o1[wb][lane] = o1q;
+      __builtin_amdgcn_wave_barrier();
+      while (R) {
+        const int q = __ffsll((long long)R) - 1;
+        R &= R - 1;
+        const int32_t v = __builtin_amdgcn_readfirstlane(h_v[wb][q]);
+        const int j = __builtin_amdgcn_readfirstlane(h_j[wb][q]);
+        const u64 L = rl64(h_L[wb][q], 0);
+        const int64_t o0 = (int64_t)rl64((u64)h_o0[wb][q], 0), o1 = (int64_t)rl64((u64)h_o1[wb][q], 0);
+        back_one<NW>(g, b, v, j, L, o0, o1, d, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
+      }
+      __builtin_amdgcn_wave_barrier();
+#else
       while (R) {
         const int q = __ffsll((long long)R) - 1;
         R &= R - 1;
         back_one<NW>(g, b, rl32(vq, q), rl32(jq, q), rl64(Lq, q), (int64_t)rl64((u64)o0q, q),
                       (int64_t)rl64((u64)o1q, q), d, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
       }
+#endif
     }
   }
   {
@@ -2143,10 +2214,16 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b) {
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BACK_WAVES))) k_ms_back(GA g, BA b, KT* kt) {
   kt_enter(kt);
+#if BACK_HDR_LDS
+  __shared__ RecHdr hdr;
+  RecHdr* hp = &hdr;
+#else
+  RecHdr* hp = nullptr;
+#endif
   const int nw = b.ctl->narrow;
-  if (nw == 2) k_ms_back_body<2>(g, b);
-  else if (nw == 1) k_ms_back_body<1>(g, b);
-  else k_ms_back_body<0>(g, b);
+  if (nw == 2) k_ms_back_body<2>(g, b, hp);
+  else if (nw == 1) k_ms_back_body<1>(g, b, hp);
+  else k_ms_back_body<0>(g, b, hp);
   kt_exit(kt);
 }
 
