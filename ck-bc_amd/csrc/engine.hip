@@ -165,6 +165,9 @@ __device__ __forceinline__ u64 ld_coherent_u64(const u64* p) {
 #ifndef BU_HDR_LDS
 #define BU_HDR_LDS 1  // bottom-up record headers held in LDS instead of registers
 #endif
+#ifndef BACK_PK
+#define BACK_PK 1  // small backward records several to a wave (needs BACK_HDR_LDS)
+#endif
 #ifndef BACK_HDR_LDS
 #define BACK_HDR_LDS 1  // backward record headers held in LDS instead of registers
 #endif
@@ -2113,6 +2116,92 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
   }
 }
 
+// Backward level d for up to BPK_W records of at most BPK_DEG arcs (a vertex
+// of one record each), given by their header slots pick[] (-1: none):
+// sub-group k (lanes k*BPK_DEG ..) loads the arcs of record k and probes their
+// ends' level d+1 masks, one arc per lane; the row gathers then run with
+// lanes = sources over the successors of all of them, each record with its
+// own accumulator, its terms added in adjacency order as back_one adds them.
+#ifndef BPK_W
+#define BPK_W 2
+#endif
+#define BPK_DEG (64 / BPK_W)
+#ifndef BPK_MAX
+#define BPK_MAX BPK_DEG  // records of at most this many arcs take the packed path
+#endif
+#ifndef UNR_BPK
+#define UNR_BPK 3  // row gathers in flight per wave in back_pk
+#endif
+template <int NW>
+__device__ __forceinline__ void back_pk(const GA& g, const BA& b, const RecHdr* hdr, int wb, const int (&pick)[BPK_W],
+                                        const u64* lmn, u64* lmd, const unsigned* nlb, unsigned* dlb,
+                                        double mult, bool scaled, double inv_md) {
+  const int lane = lane_id();
+  const int sub = lane / BPK_DEG, sl = lane % BPK_DEG;
+  int32_t vr[BPK_W];
+  u64 Lr[BPK_W];
+  int my = -1;
+#pragma unroll
+  for (int r = 0; r < BPK_W; r++) {
+    const int k = pick[r] < 0 ? 0 : pick[r];
+    vr[r] = __builtin_amdgcn_readfirstlane(hdr->v[wb][k]);
+    Lr[r] = pick[r] < 0 ? 0ull : rl64(hdr->L[wb][k], 0);
+    if (sub == r) my = pick[r];
+  }
+  int32_t z = 0;
+  u64 M = 0;
+  if (my >= 0) {
+    const int64_t o0 = hdr->o0[wb][my], o1 = hdr->o1[wb][my];
+    const u64 Ls = hdr->L[wb][my];
+    if (sl < o1 - o0) {
+      z = adj_at(g, g.adj, o0 + sl);
+      if (!(g.flags & KF_GATE) || lb_test(nlb, z)) M = lmn[z] & Ls;
+    }
+  }
+  double svr[BPK_W], acc[BPK_W];
+#pragma unroll
+  for (int r = 0; r < BPK_W; r++) {
+    const bool mine = (Lr[r] >> lane) & 1;
+    const u64 sbits = mine ? sg_ld<NW>(b.sigma, (int64_t)vr[r] * 64 + lane) : 1ull;
+    svr[r] = !mine ? 1.0 : scaled ? __longlong_as_double((long long)sbits) : (double)sbits;
+    acc[r] = (pick[r] >= 0 && g.w) ? (double)g.w[vr[r]] : 0.0;
+  }
+  for_lanes<UNR_BPK>(__ballot(M != 0), [&](const int (&q)[UNR_BPK], const bool (&ok)[UNR_BPK]) {
+    double x[UNR_BPK];
+#pragma unroll
+    for (int t = 0; t < UNR_BPK; t++) {
+      const int32_t zq = rl32(z, q[t]);
+      const u64 mq = ok[t] ? rl64(M, q[t]) : 0ull;
+      x[t] = 0.0;
+      if (in_mask(mq)) x[t] = b.coef[(int64_t)zq * 64 + lane];
+    }
+#pragma unroll
+    for (int t = 0; t < UNR_BPK; t++) {
+      const int rt = q[t] / BPK_DEG;  // wave-uniform
+#pragma unroll
+      for (int r = 0; r < BPK_W; r++)
+        if (ok[t] && r == rt) acc[r] = acc[r] + x[t] * (scaled ? svr[r] * inv_md : svr[r]);
+    }
+  });
+#pragma unroll
+  for (int r = 0; r < BPK_W; r++) {
+    if (pick[r] < 0) continue;
+    const int32_t v = vr[r];
+    double contrib = 0.0;
+    if ((Lr[r] >> lane) & 1) {
+      b.coef[(int64_t)v * 64 + lane] = (acc[r] + 1.0) / svr[r];
+      contrib = mult * acc[r];
+      if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = acc[r];
+    }
+    contrib = wave_sum_d(contrib);
+    if (lane == 0) {
+      b.bc[v] += contrib;
+      lmd[v] = Lr[r];
+      if (g.flags & KF_GATE) lb_set(dlb, v);
+    }
+  }
+}
+
 template <int NW>
 __device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
   Ctl* c = b.ctl;
@@ -2173,6 +2262,23 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
 #if BACK_HDR_LDS
       h_v[wb][lane] = vq; h_j[wb][lane] = jq; h_L[wb][lane] = Lq; h_o0[wb][lane] = o0q; h_o1[wb][lane] = o1q;
       __builtin_amdgcn_wave_barrier();
+#if BACK_PK
+      {
+        // records of at most BPK_MAX arcs, BPK_W at a time
+        const bool small = rq < hi && jq == 0 && o1q - o0q <= BPK_MAX && o1q - o0q <= g.chunk;
+        u64 P = __ballot(small);
+        R &= ~P;
+        while (P) {
+          int pick[BPK_W];
+#pragma unroll
+          for (int t = 0; t < BPK_W; t++) {
+            pick[t] = P ? __ffsll((long long)P) - 1 : -1;
+            P &= P - 1;
+          }
+          back_pk<NW>(g, b, hdr, wb, pick, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
+        }
+      }
+#endif
       while (R) {
         const int q = __ffsll((long long)R) - 1;
         R &= R - 1;
