@@ -1437,7 +1437,9 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 #ifndef UNR_BU
 #define UNR_BU 12  // parent gathers in flight per wave in the bottom-up kernel
 #endif
+#ifndef UNR_PK
 #define UNR_PK 4  // parent gathers in flight per wave in the packed bottom-up kernel
+#endif
 #define PACK_DEG 16  // packed bottom-up: vertices of degree <= PACK_DEG ...
 #define PACK_W 4     // ... PACK_W to a wave (PACK_DEG * PACK_W = 64)
 static_assert(PACK_DEG * PACK_W == 64 && PACK_W == 4, "the packed bottom-up scan picks one vertex per sub-group of 16 lanes");
@@ -1660,6 +1662,12 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
         open = svq != b.compmask[g.label[vq]];
       }
       u64 O = __ballot(open);
+#if BU_HDR_LDS
+      const int wb = threadIdx.x >> 6;
+      sh->hdr.v[wb][lane] = vq; sh->hdr.L[wb][lane] = svq;
+      sh->hdr.o0[wb][lane] = o0q; sh->hdr.o1[wb][lane] = o1q;
+      __builtin_amdgcn_wave_barrier();
+#endif
       while (O) {
         // the next PACK_W unfinished vertices, one per sub-group (-1: none)
         int pick[PACK_W];
@@ -1669,12 +1677,21 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
           O &= O - 1;
         }
         const int my = sub == 0 ? pick[0] : sub == 1 ? pick[1] : sub == 2 ? pick[2] : pick[3];
+#if BU_HDR_LDS
+        const int myk = my < 0 ? 0 : my;
+        const int32_t v = sh->hdr.v[wb][myk];
+        const u64 svs = sh->hdr.L[wb][myk];
+        const u64 sv = my < 0 ? 0ull : svs;
+        const int64_t o0s = sh->hdr.o0[wb][myk];
+        const int64_t o1s = sh->hdr.o1[wb][myk];
+#else
         // both shuffles run on the whole wave (a source lane must be active)
         const int32_t v = __shfl(vq, my < 0 ? 0 : my, 64);
         const u64 svs = (u64)__shfl((unsigned long long)svq, my < 0 ? 0 : my, 64);
         const u64 sv = my < 0 ? 0ull : svs;
         const int64_t o0s = (int64_t)__shfl((unsigned long long)o0q, my < 0 ? 0 : my, 64);
         const int64_t o1s = (int64_t)__shfl((unsigned long long)o1q, my < 0 ? 0 : my, 64);
+#endif
         const int64_t e0 = my < 0 ? 0 : o0s, dgv = my < 0 ? 0 : o1s - o0s;
         int32_t u = 0;
         u64 need = 0;
@@ -1758,6 +1775,9 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
         }
         app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
       }
+#if BU_HDR_LDS
+      __builtin_amdgcn_wave_barrier();
+#endif
     }
   }
   app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
@@ -1779,8 +1799,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BU_WAV
 }
 
 // The packed low-degree vertices, in a kernel of their own (own register budget:
-// 5 waves per SIMD, 96 VGPRs; its load chains need the extra resident wave).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_ms_bu_pk(GA g, BA b, KT* kt) {
+// 6 waves per SIMD, 80 VGPRs and a few spilled outside the gather loop; its
+// load chains need the extra resident waves).
+#ifndef BU_PK_WAVES
+#define BU_PK_WAVES 6
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BU_PK_WAVES))) k_ms_bu_pk(GA g, BA b, KT* kt) {
   kt_enter(kt);
   __shared__ BuLds sh;
   const int nw = b.ctl->narrow;
