@@ -2263,11 +2263,11 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
     // the headers wait in LDS (a wave reads back only its own slots, and a
     // wave's LDS accesses complete in issue order), which leaves their
     // registers to the row gathers
-    int32_t (&h_v)[4][64] = hdr->v;
-    int32_t (&h_j)[4][64] = hdr->j;
-    u64 (&h_L)[4][64] = hdr->L;
-    int64_t (&h_o0)[4][64] = hdr->o0;
-    int64_t (&h_o1)[4][64] = hdr->o1;
+    int32_t (&h_v)[MAX_WAVES_PER_BLOCK][64] = hdr->v;
+    int32_t (&h_j)[MAX_WAVES_PER_BLOCK][64] = hdr->j;
+    u64 (&h_L)[MAX_WAVES_PER_BLOCK][64] = hdr->L;
+    int64_t (&h_o0)[MAX_WAVES_PER_BLOCK][64] = hdr->o0;
+    int64_t (&h_o1)[MAX_WAVES_PER_BLOCK][64] = hdr->o1;
     const int wb = threadIdx.x >> 6;
 #endif
     for (int64_t k0 = 0; lo + gw + k0 * tw < hi; k0 += 64) {
