@@ -162,14 +162,8 @@ __device__ __forceinline__ u64 ld_coherent_u64(const u64* p) {
 #ifndef UNR_BACK
 #define UNR_BACK UNR  // row gathers in flight per wave in the backward loop
 #endif
-#ifndef BU_HDR_LDS
-#define BU_HDR_LDS 1  // bottom-up record headers held in LDS instead of registers
-#endif
 #ifndef BACK_PK
-#define BACK_PK 1  // small backward records several to a wave (needs BACK_HDR_LDS)
-#endif
-#ifndef BACK_HDR_LDS
-#define BACK_HDR_LDS 1  // backward record headers held in LDS instead of registers
+#define BACK_PK 1  // small backward records several to a wave
 #endif
 #define KF_GATE 1  // gate mask probes with the level bitmaps
 #define KF_NT 2    // non-temporal adjacency loads
@@ -1444,7 +1438,7 @@ __global__ void __launch_bounds__(256) k_ms_td(GA g, BA b, KT* kt) {
 #define PACK_W 4     // ... PACK_W to a wave (PACK_DEG * PACK_W = 64)
 static_assert(PACK_DEG * PACK_W == 64 && PACK_W == 4, "the packed bottom-up scan picks one vertex per sub-group of 16 lanes");
 // Headers of 64 records for each of a block's four waves, held in LDS while
-// the wave runs the records (BACK_HDR_LDS, BU_HDR_LDS): vertex, record index,
+// the wave runs the records (bottom-up and backward kernels): vertex, record index,
 // a mask, first and end arc.
 struct RecHdr {
   int32_t v[MAX_WAVES_PER_BLOCK][64], j[MAX_WAVES_PER_BLOCK][64];
@@ -1502,28 +1496,19 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
       open = svq != b.compmask[g.label[vq]];
     }
     u64 O = __ballot(open);
-#if BU_HDR_LDS
     // the headers wait in LDS (a wave reads back only its own slots, and a
     // wave's LDS accesses complete in issue order)
     const int wb = threadIdx.x >> 6;
     sh->hdr.v[wb][lane] = vq; sh->hdr.j[wb][lane] = jq; sh->hdr.L[wb][lane] = svq;
     sh->hdr.o0[wb][lane] = o0q; sh->hdr.o1[wb][lane] = o1q;
     __builtin_amdgcn_wave_barrier();
-#endif
     while (O) {
       const int oq = __ffsll((long long)O) - 1;
       O &= O - 1;
-#if BU_HDR_LDS
       const int32_t v = __builtin_amdgcn_readfirstlane(sh->hdr.v[wb][oq]);
       const u64 sv = rl64(sh->hdr.L[wb][oq], 0);
       const int j = __builtin_amdgcn_readfirstlane(sh->hdr.j[wb][oq]);
       const int64_t o0 = (int64_t)rl64((u64)sh->hdr.o0[wb][oq], 0), o1 = (int64_t)rl64((u64)sh->hdr.o1[wb][oq], 0);
-#else
-      const int32_t v = rl32(vq, oq);
-      const u64 sv = rl64(svq, oq);
-      const int j = rl32(jq, oq);
-      const int64_t o0 = (int64_t)rl64((u64)o0q, oq), o1 = (int64_t)rl64((u64)o1q, oq);
-#endif
       const int64_t deg = o1 - o0;
       int64_t e0 = o0 + (int64_t)j * g.chunk;
       int64_t e1 = min(o1, e0 + g.chunk);
@@ -1634,9 +1619,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
       }
       app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
     }
-#if BU_HDR_LDS
     __builtin_amdgcn_wave_barrier();
-#endif
   }
   if (PACKED) {
     // Vertices of degree <= PACK_DEG, PACK_W per wave: sub-group q (lanes
@@ -1662,12 +1645,10 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
         open = svq != b.compmask[g.label[vq]];
       }
       u64 O = __ballot(open);
-#if BU_HDR_LDS
       const int wb = threadIdx.x >> 6;
       sh->hdr.v[wb][lane] = vq; sh->hdr.L[wb][lane] = svq;
       sh->hdr.o0[wb][lane] = o0q; sh->hdr.o1[wb][lane] = o1q;
       __builtin_amdgcn_wave_barrier();
-#endif
       while (O) {
         // the next PACK_W unfinished vertices, one per sub-group (-1: none)
         int pick[PACK_W];
@@ -1677,21 +1658,12 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
           O &= O - 1;
         }
         const int my = sub == 0 ? pick[0] : sub == 1 ? pick[1] : sub == 2 ? pick[2] : pick[3];
-#if BU_HDR_LDS
         const int myk = my < 0 ? 0 : my;
         const int32_t v = sh->hdr.v[wb][myk];
         const u64 svs = sh->hdr.L[wb][myk];
         const u64 sv = my < 0 ? 0ull : svs;
         const int64_t o0s = sh->hdr.o0[wb][myk];
         const int64_t o1s = sh->hdr.o1[wb][myk];
-#else
-        // both shuffles run on the whole wave (a source lane must be active)
-        const int32_t v = __shfl(vq, my < 0 ? 0 : my, 64);
-        const u64 svs = (u64)__shfl((unsigned long long)svq, my < 0 ? 0 : my, 64);
-        const u64 sv = my < 0 ? 0ull : svs;
-        const int64_t o0s = (int64_t)__shfl((unsigned long long)o0q, my < 0 ? 0 : my, 64);
-        const int64_t o1s = (int64_t)__shfl((unsigned long long)o1q, my < 0 ? 0 : my, 64);
-#endif
         const int64_t e0 = my < 0 ? 0 : o0s, dgv = my < 0 ? 0 : o1s - o0s;
         int32_t u = 0;
         u64 need = 0;
@@ -1775,9 +1747,7 @@ __device__ __forceinline__ void k_ms_bu_body(GA g, BA b, BuLds* sh) {
         }
         app_push(ap, won, v, dg, g.chunk, c, b.rec_v, b.rec_j, b.cap);
       }
-#if BU_HDR_LDS
       __builtin_amdgcn_wave_barrier();
-#endif
     }
   }
   app_finish(ap, c, b.rec_v, b.rec_j, b.cap);
@@ -2259,7 +2229,6 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
   } else {
     // This wave's records lo + gw, lo + gw + tw, ...: the headers of 64 of
     // them load at once, one lane each, then the records run in order.
-#if BACK_HDR_LDS
     // the headers wait in LDS (a wave reads back only its own slots, and a
     // wave's LDS accesses complete in issue order), which leaves their
     // registers to the row gathers
@@ -2269,7 +2238,6 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
     int64_t (&h_o0)[MAX_WAVES_PER_BLOCK][64] = hdr->o0;
     int64_t (&h_o1)[MAX_WAVES_PER_BLOCK][64] = hdr->o1;
     const int wb = threadIdx.x >> 6;
-#endif
     for (int64_t k0 = 0; lo + gw + k0 * tw < hi; k0 += 64) {
       const int64_t rq = lo + gw + (k0 + lane) * tw;
       int32_t vq = 0, jq = 0;
@@ -2283,7 +2251,6 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
         o1q = g.off[vq + 1];
       }
       u64 R = __ballot(rq < hi);
-#if BACK_HDR_LDS
       h_v[wb][lane] = vq; h_j[wb][lane] = jq; h_L[wb][lane] = Lq; h_o0[wb][lane] = o0q; h_o1[wb][lane] = o1q;
       __builtin_amdgcn_wave_barrier();
 #if BACK_PK
@@ -2313,14 +2280,6 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
         back_one<NW>(g, b, v, j, L, o0, o1, d, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
       }
       __builtin_amdgcn_wave_barrier();
-#else
-      while (R) {
-        const int q = __ffsll((long long)R) - 1;
-        R &= R - 1;
-        back_one<NW>(g, b, rl32(vq, q), rl32(jq, q), rl64(Lq, q), (int64_t)rl64((u64)o0q, q),
-                      (int64_t)rl64((u64)o1q, q), d, lmn, lmd, nlb, dlb, mult, scaled, inv_md);
-      }
-#endif
     }
   }
   {
@@ -2344,16 +2303,11 @@ __device__ __forceinline__ void k_ms_back_body(GA g, BA b, RecHdr* hdr) {
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BACK_WAVES))) k_ms_back(GA g, BA b, KT* kt) {
   kt_enter(kt);
-#if BACK_HDR_LDS
   __shared__ RecHdr hdr;
-  RecHdr* hp = &hdr;
-#else
-  RecHdr* hp = nullptr;
-#endif
   const int nw = b.ctl->narrow;
-  if (nw == 2) k_ms_back_body<2>(g, b, hp);
-  else if (nw == 1) k_ms_back_body<1>(g, b, hp);
-  else k_ms_back_body<0>(g, b, hp);
+  if (nw == 2) k_ms_back_body<2>(g, b, &hdr);
+  else if (nw == 1) k_ms_back_body<1>(g, b, &hdr);
+  else k_ms_back_body<0>(g, b, &hdr);
   kt_exit(kt);
 }
 
