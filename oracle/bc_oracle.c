@@ -19,6 +19,10 @@
  *   - degree-1 reduction is iterative (reference: a single pass, c:314-419),
  *     with exact BC for pruned vertices (reference leaves them at 0).
  *
+ * Past 2^64, orc_brandes_scaled keeps path counts as binary64 values with a
+ * power-of-two scale per level (not exact; pinned against the big-integer
+ * Brandes of oracle.py and against orc_brandes where counts fit).
+ *
  * Parity pins (tests/test_oracle.py): closed forms (paths, stars, cycles,
  * cliques, grids, hypercubes incl. sigma overflow), brute-force all-pairs path
  * counting on small random graphs, networkx 3.4.2 golden vectors (facebook
@@ -30,6 +34,7 @@
 #include <string.h>
 #include <stdio.h>
 #include <ctype.h>
+#include <math.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -592,6 +597,110 @@ int orc_depth_sums(int64_t n, const int64_t *off, const int32_t *adj, const int3
         free(q);
     }
     return err ? -ORC_ERR_NOMEM : 0;
+}
+
+/* Brandes with path counts past 2^64 (and past binary64's range): sigma is
+ * held as a binary64 value per vertex times a power of two per level,
+ * sigma(v) = sig[v] * 2^E[depth v].  After each forward level the level's
+ * counts are rescaled by a power of two (exact) so that the largest lies in
+ * [1, 2).  Same recurrences as sssp (cu:524-537, cu:590-596):
+ * sigma[r] += sigma[c] over arcs to the next level, and
+ * delta[c] += (delta[r] + 1) / sigma[r] * sigma[c] in adjacency order, where
+ * sigma[c] / sigma[r] = sig[c] / sig[r] * 2^(E[d] - E[d+1]).  Counts above
+ * 2^53 round, so the result carries binary64 rounding over the levels rather
+ * than being exact; used on long bridged-block chains, where orc_brandes
+ * reports overflow.  Ordered-pair BC added into bc (not halved); OpenMP over
+ * sources with thread-private partials summed in thread order. */
+int orc_brandes_scaled(int64_t n, const int64_t *off, const int32_t *adj, const int32_t *src, int64_t nsrc,
+                       double *bc, int nthreads) {
+    int nt = nthreads > 0 ? nthreads : 1;
+    double *part = (double *)calloc((size_t)nt * (size_t)(n > 0 ? n : 1), sizeof(double));
+    if (!part) return ORC_ERR_NOMEM;
+    int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+        double *my = part + (size_t)tid * n;
+        size_t nn = (size_t)(n > 0 ? n : 1);
+        int32_t *depth = (int32_t *)malloc(nn * sizeof(int32_t));
+        double *sig = (double *)malloc(nn * sizeof(double));
+        double *delta = (double *)malloc(nn * sizeof(double));
+        int32_t *q = (int32_t *)malloc((nn + 1) * sizeof(int32_t));
+        int64_t *lo = (int64_t *)malloc((nn + 2) * sizeof(int64_t));
+        int *shift = (int *)malloc((nn + 2) * sizeof(int));  /* E[d] - E[d-1] */
+        if (!depth || !sig || !delta || !q || !lo || !shift) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+            bad = ORC_ERR_NOMEM;
+        } else {
+            for (int64_t v = 0; v < n; v++) depth[v] = -1;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+            for (int64_t i = 0; i < nsrc; i++) {
+                int32_t s = src[i];
+                if (off[s + 1] == off[s]) continue;
+                int64_t nq = 0, L = 0;
+                q[nq++] = s; depth[s] = 0; sig[s] = 1.0;
+                lo[0] = 0; lo[1] = 1; shift[0] = 0;
+                for (;;) {
+                    int64_t b = lo[L], e = lo[L + 1];
+                    if (b == e) break;
+                    int32_t d1 = (int32_t)(L + 1);
+                    for (int64_t k = b; k < e; k++) {
+                        int32_t c = q[k];
+                        for (int64_t a = off[c]; a < off[c + 1]; a++) {
+                            int32_t r = adj[a];
+                            if (depth[r] == -1) { depth[r] = d1; sig[r] = 0.0; q[nq++] = r; }
+                            if (depth[r] == d1) sig[r] = sig[r] + sig[c];
+                        }
+                    }
+                    L++;
+                    lo[L + 1] = nq;
+                    /* rescale the new level so that its largest count is in [1, 2) */
+                    double mx = 0.0;
+                    for (int64_t k = lo[L]; k < nq; k++) mx = sig[q[k]] > mx ? sig[q[k]] : mx;
+                    int ex = 0;
+                    if (mx > 0.0) { (void)frexp(mx, &ex); ex -= 1; }
+                    shift[L] = ex;
+                    for (int64_t k = lo[L]; k < nq; k++) sig[q[k]] = ldexp(sig[q[k]], -ex);
+                }
+                for (int64_t k = 0; k < nq; k++) delta[q[k]] = 0.0;
+                for (int64_t d = L - 2; d >= 1; d--) {
+                    for (int64_t k = lo[d]; k < lo[d + 1]; k++) {
+                        int32_t c = q[k];
+                        double acc = delta[c];
+                        double sc = ldexp(sig[c], -shift[d + 1]);  /* sigma[c] in level d+1's units */
+                        for (int64_t a = off[c]; a < off[c + 1]; a++) {
+                            int32_t r = adj[a];
+                            if (depth[r] == d + 1) {
+                                double t = (delta[r] + 1.0) / sig[r];
+                                acc = acc + t * sc;
+                            }
+                        }
+                        delta[c] = acc;
+                    }
+                }
+                for (int64_t k = 1; k < nq; k++) my[q[k]] += delta[q[k]];
+                for (int64_t k = 0; k < nq; k++) depth[q[k]] = -1;
+            }
+        }
+        free(depth); free(sig); free(delta); free(q); free(lo); free(shift);
+    }
+    for (int64_t v = 0; v < n; v++) {
+        double acc = 0.0;
+        for (int t = 0; t < nt; t++) acc += part[(size_t)t * n + v];
+        bc[v] += acc;
+    }
+    free(part);
+    return bad;
 }
 
 int orc_version(void) { return 1; }

@@ -56,6 +56,8 @@ def lib():
                                       ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int]
         L.orc_depth_sums.restype = ctypes.c_int
         L.orc_depth_sums.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p, ctypes.c_int64, ctypes.c_int, _i64p]
+        L.orc_brandes_scaled.restype = ctypes.c_int
+        L.orc_brandes_scaled.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p, ctypes.c_int64, _f64p, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -189,6 +191,20 @@ def depth_sums(off, adj, sources, nthreads=1):
     if rc != 0:
         raise OracleError(f"orc_depth_sums rc={rc}")
     return out
+
+
+def brandes_scaled(off, adj, sources, nthreads=1):
+    """Ordered-pair BC (not halved) over the sources with path counts as
+    binary64 values and a power-of-two scale per BFS level, for graphs whose
+    counts pass 2^64 (long bridged-block chains).  Not exact: binary64
+    rounding over the levels (orc_brandes_scaled)."""
+    n = off.shape[0] - 1
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    bc = np.zeros(n, np.float64)
+    rc = lib().orc_brandes_scaled(n, off, adj, src, len(src), bc, nthreads)
+    if rc != 0:
+        raise OracleError(f"orc_brandes_scaled rc={rc}")
+    return bc
 
 
 def brandes_highprec(off, adj, sources, digits=50):

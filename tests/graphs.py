@@ -46,6 +46,20 @@ def diamonds(k):
     return np.array(e), 3 * k + 1
 
 
+def block_chain(nblocks, block, avgdeg, seed):
+    """Bridged-block chain: nblocks random blocks of `block` vertices with
+    about block*avgdeg/2 edges each, consecutive blocks joined by one edge
+    between random vertices of each (path counts multiply along the chain)."""
+    rng = np.random.default_rng(seed)
+    m = int(block * avgdeg / 2)
+    e = []
+    for b in range(nblocks):
+        e.append(b * block + rng.integers(0, block, size=(m, 2)))
+        if b + 1 < nblocks:
+            e.append(np.array([[b * block + rng.integers(0, block), (b + 1) * block + rng.integers(0, block)]]))
+    return np.concatenate(e), nblocks * block
+
+
 def random_graph(n, m, seed):
     rng = np.random.default_rng(seed)
     return rng.integers(0, n, size=(m, 2)), n

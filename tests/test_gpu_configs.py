@@ -186,6 +186,10 @@ def test_rmat_scale25():
 
 
 def test_chain_32768_blocks():
+    """The 8.4M-vertex bridged-block chain (diameter past 30,000, path counts
+    past 2^64 and past binary64's range): 256 sources through workgroup
+    sweeps, their BC total against the oracle's distance checksum, and 16 of
+    them vertex by vertex against the oracle's scaled-count Brandes."""
     t = time.time()
     g = bcmi.Graph.block_chain(32768, 256, 8.0, seed=SEED)
     src = g.sample_sources(1024, seed=SEED + 1)
@@ -204,4 +208,11 @@ def test_chain_32768_blocks():
     assert np.all(np.isfinite(bc)) and np.all(bc >= 0)
     ds = O.depth_sums(g.off, g.adj, sub, nthreads=NTH)
     assert rel_sum(bc.sum(), ds.sum()) <= 1e-9
+    few = sub[:16]
+    e.reset()
+    e.run(few, mode=bcmi.MODE_BATCHED)
+    t = time.time()
+    ref = O.brandes_scaled(g.off, g.adj, few, nthreads=NTH)
+    say(f"chain32768: scaled-count oracle on 16 sources in {time.time() - t:.1f}s")
+    assert bcmi.rel_err(e.bc(), ref) <= 1e-9
     e.close()

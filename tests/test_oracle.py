@@ -284,6 +284,38 @@ def test_highprec_oracle_matches_uint64_oracle():
     assert bc[1] == bc[2] and bc[1] > 0
 
 
+def rel_err(a, b):
+    """max |a-b|/|b| over entries with b != 0; entries with b == 0 must be 0."""
+    nz = b != 0
+    if np.any(a[~nz] != 0):
+        return float("inf")
+    return float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz]))) if nz.any() else 0.0
+
+
+def test_scaled_oracle_matches_exact_and_highprec():
+    """orc_brandes_scaled (binary64 path counts with a power-of-two scale per
+    level) equals the uint64 oracle where the counts fit (to binary64
+    rounding), and the big-integer Brandes past 2^64: on a bridged-block
+    chain and on a diamond chain of 2^1100 paths, beyond binary64's range,
+    where an unscaled binary64 count would overflow to infinity."""
+    for seed in range(3):
+        off, adj = csr(*G.random_graph(200, 600, seed))
+        src = np.arange(0, 200, 5)
+        assert rel_err(O.brandes_scaled(off, adj, src, nthreads=4), O.brandes(off, adj, src)) <= 1e-13
+    off, adj = csr(*G.block_chain(200, 32, 8.0, 5))
+    src = [3, 3200, 6390]
+    with pytest.raises(O.OracleError):
+        O.brandes(off, adj, src)                        # counts beyond 2^64
+    hp = O.brandes_highprec(off, adj, src)
+    assert rel_err(O.brandes_scaled(off, adj, src, nthreads=2), hp) <= 1e-12
+    off, adj = csr(*G.diamonds(1100))
+    src = [0, 7, 1650]
+    hp = O.brandes_highprec(off, adj, src)
+    sc = O.brandes_scaled(off, adj, src)
+    assert np.all(np.isfinite(sc))
+    assert rel_err(sc, hp) <= 1e-12
+
+
 def test_depth_sums_closed_forms_and_bc_identity():
     """Distance checksum: on a path from an end it is (n-1)(n-2)/2; from the
     centre of a star 0; on any graph it equals the sum of the sources'
