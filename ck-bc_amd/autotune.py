@@ -50,7 +50,7 @@ SPACE = {
     "steps": [6, 8, 10, 12, 16],
     "width": [32, 48, 64],
     "narrow_sigma": [1, 2],
-    "concurrency": [0, 1, 2, 3, 4],
+    "concurrency": [0, 1, 2, 3, 4, 6, 8],
 }
 PLAIN_SPACE = {"alpha_plain": [100, 300, 1000, 3000, 10000, 1 << 60], "grid_plain": [1024, 2048, 4096, 8192]}
 DEFAULTS = {"chunk": 256, "block": 256, "grid": 4096, "grid_back": 0, "alpha_batched": 1000, "beta_batched": 16,
@@ -72,9 +72,23 @@ def graph_key(kind, **kw):
 class Tuner:
     """Measures batched and plain throughput of one graph under parameter sets."""
 
-    def __init__(self, g, sources, batches, plain, heuristic=bcmi.H_NONE, log=print):
+    def __init__(self, g, sources, batches, plain, heuristic=bcmi.H_NONE, log=print, min_time=0.3):
         self.g, self.sources, self.batches, self.plain, self.h, self.log = g, sources, batches, plain, heuristic, log
         self.e, self.chunk, self.width = None, None, None
+        # a measurement repeats its run until it has timed at least min_time
+        # seconds (small graphs: one run of a few milliseconds is mostly noise)
+        self.min_time = min_time
+
+    def _rate(self, e, src, mode):
+        te = t_sum = 0.0
+        while True:
+            e.reset()
+            t = time.perf_counter()
+            st = e.run(src, mode=mode)
+            t_sum += time.perf_counter() - t
+            te += st["te"]
+            if t_sum >= self.min_time:
+                return te / t_sum / 1e9
 
     def _engine(self, p):
         """An engine holding the graph uploaded with p's record size, p applied."""
@@ -109,19 +123,13 @@ class Tuner:
         e = self._engine(p)
         warm, timed = self._timed_sources(e, p["width"])
         e.run(warm, mode=bcmi.MODE_BATCHED)  # captures the level loops for p
-        e.reset()
-        t = time.perf_counter()
-        st = e.run(timed, mode=bcmi.MODE_BATCHED)
-        return st["te"] / (time.perf_counter() - t) / 1e9
+        return self._rate(e, timed, bcmi.MODE_BATCHED)
 
     def plain_rate(self, p):
         e = self._engine(p)
         src = self.sources[:self.plain]
         e.run(src[:2], mode=bcmi.MODE_PLAIN)
-        e.reset()
-        t = time.perf_counter()
-        st = e.run(src, mode=bcmi.MODE_PLAIN)
-        return st["te"] / (time.perf_counter() - t) / 1e9
+        return self._rate(e, src, bcmi.MODE_PLAIN)
 
     def close(self):
         if self.e is not None:
@@ -160,14 +168,14 @@ def descend(measure, start, space, passes, min_gain, log, tag):
 
 
 def tune(g, sources, batches=32, plain=16, start=None, space=None, plain_space=None, passes=2, min_gain=0.005,
-         heuristic=bcmi.H_NONE, log=print):
+         heuristic=bcmi.H_NONE, log=print, min_time=0.3):
     """Returns the tuned parameters (APPLIED keys) and the measured GTEPS."""
     space = SPACE if space is None else space
     plain_space = PLAIN_SPACE if plain_space is None else plain_space
     p = dict(DEFAULTS)
     if start:
         p.update({k: int(v) for k, v in start.items() if k in APPLIED})
-    T = Tuner(g, sources, batches, plain, heuristic, log)
+    T = Tuner(g, sources, batches, plain, heuristic, log, min_time)
     try:
         p, score = descend(T.batched, p, space, passes, min_gain, log, "batched")
         p, pscore = descend(T.plain_rate, p, plain_space, passes, min_gain, log, "plain")
@@ -207,6 +215,7 @@ def main():
     ap.add_argument("--plain", type=int, default=16, help="sources per plain measurement")
     ap.add_argument("--passes", type=int, default=2)
     ap.add_argument("--min-gain", type=float, default=0.005)
+    ap.add_argument("--min-time", type=float, default=0.3, help="seconds each measurement times at least (runs repeat)")
     ap.add_argument("--fresh", action="store_true", help="start from the engine defaults, not the table entry")
     ap.add_argument("--out", default=DEFAULT_PATH)
     a = ap.parse_args()
@@ -222,7 +231,7 @@ def main():
     start = None if a.fresh else table.get(key)
     t0 = time.time()
     p, score, pscore = tune(g, src, a.batches, a.plain, start, passes=a.passes, min_gain=a.min_gain,
-                            heuristic=heur, log=log)
+                            heuristic=heur, log=log, min_time=a.min_time)
     entry = {k: int(p[k]) for k in APPLIED}
     entry.update({
         "batched_gteps": round(score, 2), "plain_gteps": round(pscore, 3), "tuned_on": time.strftime("%Y-%m-%d"),
@@ -230,7 +239,7 @@ def main():
         "command": "python ck-bc_amd/autotune.py " + " ".join(sys.argv[1:]),
         "sample": (f"R-MAT scale {a.scale} ef {a.edgefactor}" + (f" + {a.pendant:.0%} pendant trees, peeled" if heur else "")
                    + f", seed {a.seed}; {a.batches} batches of the {a.sources}-source workload per measurement, "
-                   f"{a.plain} plain sweeps; {a.passes} passes from "
+                   f"{a.plain} plain sweeps, each run repeated to at least {a.min_time:g} s; {a.passes} passes from "
                    + ("the defaults" if start is None else "the previous entry")
                    + f"; {time.time() - t0:.0f} s")})
     table[key] = entry
