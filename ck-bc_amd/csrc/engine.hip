@@ -2102,7 +2102,10 @@ __device__ __forceinline__ void back_one(const GA& g, const BA& b, int32_t v, in
       if (b.dbg_delta) b.dbg_delta[(int64_t)v * 64 + lane] = acc;
     }
     contrib = wave_sum_d(contrib);
-    if (lane == 0) b.bc[v] += contrib;
+    // a non-returning atomic add leaves no load of bc[v] to wait for; a
+    // vertex held in one record is the only adder of bc[v] at this level,
+    // so the sum is the one a load and store would give
+    if (lane == 0) atomicAdd(&b.bc[v], contrib);
   }
   if (j == 0 && lane == 0) {
     lmd[v] = L;
@@ -2189,7 +2192,7 @@ __device__ __forceinline__ void back_pk(const GA& g, const BA& b, const RecHdr* 
     }
     contrib = wave_sum_d(contrib);
     if (lane == 0) {
-      b.bc[v] += contrib;
+      atomicAdd(&b.bc[v], contrib);  // as in back_one
       lmd[v] = Lr[r];
       if (g.flags & KF_GATE) lb_set(dlb, v);
     }
