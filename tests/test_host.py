@@ -284,3 +284,34 @@ def test_ck_run_commands_use_the_tuned_shapes():
             assert k in autotune.APPLIED and int(v) == entry[k], (name, k, v, entry[k])
         seen += 1
     assert seen >= 2
+
+
+def test_tuner_measurement_repeats_to_min_time():
+    """A tuner measurement repeats its run (resetting the engine before each)
+    until the timed runs add up to min_time, and reports total traversed
+    edges over total time: a run of a few milliseconds on a small graph is not
+    timed alone."""
+    import time
+    import autotune
+
+    class FakeEngine:
+        def __init__(self):
+            self.runs = self.resets = 0
+
+        def reset(self):
+            self.resets += 1
+
+        def run(self, src, mode):
+            self.runs += 1
+            time.sleep(0.01)
+            return {"te": 1e7}
+
+    t = autotune.Tuner(None, None, 1, 1, min_time=0.05)
+    e = FakeEngine()
+    rate = t._rate(e, [0], bcmi.MODE_BATCHED)
+    assert e.runs >= 5 and e.resets == e.runs
+    assert 0.0 < rate < 1.05          # 1e7 edges per >= 10 ms run: at most 1 GTEPS
+    t = autotune.Tuner(None, None, 1, 1, min_time=0.0)
+    e = FakeEngine()
+    t._rate(e, [0], bcmi.MODE_BATCHED)
+    assert e.runs == 1
