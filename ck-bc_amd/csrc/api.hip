@@ -167,317 +167,356 @@ const char* bcmi_last_error(void) { return last_error(); }
 int bcmi_version(void) { return 1; }
 
 int bcmi_create(int device, bcmi_ctx** out) {
-  if (!out) return set_error(BCMI_ERR_ARG, "null output");
-  int ndev = 0;
-  hipError_t e = hipGetDeviceCount(&ndev);
-  if (e != hipSuccess || ndev == 0) return set_error(BCMI_ERR_HIP, "no HIP device available");
-  if (device < 0 || device >= ndev) return set_error(BCMI_ERR_ARG, "device index out of range");
-  bcmi_ctx* c = new bcmi_ctx();
-  c->device = device;
-  int rc = use_device(c);
-  if (rc) { delete c; return rc; }
-  e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete c; return set_error(BCMI_ERR_HIP, "stream creation failed"); }
-  *out = c;
-  return BCMI_OK;
+  return guard([&]() -> int {
+    if (!out) return set_error(BCMI_ERR_ARG, "null output");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return set_error(BCMI_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return set_error(BCMI_ERR_ARG, "device index out of range");
+    bcmi_ctx* c = new bcmi_ctx();
+    c->device = device;
+    int rc = use_device(c);
+    if (rc) { delete c; return rc; }
+    e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return set_error(BCMI_ERR_HIP, "stream creation failed"); }
+    *out = c;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_destroy(bcmi_ctx* c) {
-  CHECK_CTX(c);
-  use_device(c);
-  drop_workspaces(c);
-  c->g.release();
-  if (c->bc) (void)hipFree(c->bc);
-  if (c->st) (void)hipStreamDestroy(c->st);
-  delete c;
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    use_device(c);
+    drop_workspaces(c);
+    c->g.release();
+    if (c->bc) (void)hipFree(c->bc);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_set_param(bcmi_ctx* c, const char* key, int64_t v) {
-  CHECK_CTX(c);
-  if (!key) return set_error(BCMI_ERR_ARG, "null key");
-  std::lock_guard<std::mutex> lk(c->mu);
-  return param_set(c->prm, key, v, c->have_graph);
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!key) return set_error(BCMI_ERR_ARG, "null key");
+    std::lock_guard<std::mutex> lk(c->mu);
+    return param_set(c->prm, key, v, c->have_graph);
+  });
 }
 
 int bcmi_launch_plan(const char* const* keys, const int64_t* values, int nparams, int mode, int graph,
                      bcmi_launch* out, int cap, int* count, int* steps) {
-  if (nparams < 0 || (nparams > 0 && (!keys || !values)) || !count || cap < 0 || (cap > 0 && !out))
-    return set_error(BCMI_ERR_ARG, "bad argument");
-  if (mode != BCMI_MODE_PLAIN && mode != BCMI_MODE_BATCHED) return set_error(BCMI_ERR_ARG, "bad mode");
-  if (graph < 0 || graph > 3) return set_error(BCMI_ERR_ARG, "graph must be 0..3");
-  Params p;
-  for (int i = 0; i < nparams; i++) {
-    if (!keys[i]) return set_error(BCMI_ERR_ARG, "null key");
-    int rc = param_set(p, keys[i], values[i], false);
-    if (rc) return rc;
-  }
-  const std::vector<StepNode> plan = step_plan(p, mode == BCMI_MODE_PLAIN, graph);
-  *count = (int)plan.size();
-  if (steps) *steps = p.steps;
-  for (int i = 0; i < cap && i < (int)plan.size(); i++) {
-    bcmi_launch& L = out[i];
-    memset(&L, 0, sizeof L);
-    strncpy(L.kernel, step_kernel_name(plan[i].kernel), sizeof L.kernel - 1);
-    L.grid = plan[i].grid;
-    L.block = plan[i].block;
-    L.aux = plan[i].aux;
-    L.timed = plan[i].timed ? 1 : 0;
-  }
-  return BCMI_OK;
+  return guard([&]() -> int {
+    if (nparams < 0 || (nparams > 0 && (!keys || !values)) || !count || cap < 0 || (cap > 0 && !out))
+      return set_error(BCMI_ERR_ARG, "bad argument");
+    if (mode != BCMI_MODE_PLAIN && mode != BCMI_MODE_BATCHED) return set_error(BCMI_ERR_ARG, "bad mode");
+    if (graph < 0 || graph > 3) return set_error(BCMI_ERR_ARG, "graph must be 0..3");
+    Params p;
+    for (int i = 0; i < nparams; i++) {
+      if (!keys[i]) return set_error(BCMI_ERR_ARG, "null key");
+      int rc = param_set(p, keys[i], values[i], false);
+      if (rc) return rc;
+    }
+    const std::vector<StepNode> plan = step_plan(p, mode == BCMI_MODE_PLAIN, graph);
+    *count = (int)plan.size();
+    if (steps) *steps = p.steps;
+    for (int i = 0; i < cap && i < (int)plan.size(); i++) {
+      bcmi_launch& L = out[i];
+      memset(&L, 0, sizeof L);
+      strncpy(L.kernel, step_kernel_name(plan[i].kernel), sizeof L.kernel - 1);
+      L.grid = plan[i].grid;
+      L.block = plan[i].block;
+      L.aux = plan[i].aux;
+      L.timed = plan[i].timed ? 1 : 0;
+    }
+    return BCMI_OK;
+  });
 }
 
 int bcmi_get_param(bcmi_ctx* c, const char* key, int64_t* v) {
-  CHECK_CTX(c);
-  if (!key || !v) return set_error(BCMI_ERR_ARG, "null argument");
-  std::string k(key);
-  const Params& p = c->prm;
-  if (k == "label_passes") {  // read-only: union-find passes the uploaded graph's labels took
-    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-    *v = c->g.cc_passes;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!key || !v) return set_error(BCMI_ERR_ARG, "null argument");
+    std::string k(key);
+    const Params& p = c->prm;
+    if (k == "label_passes") {  // read-only: union-find passes the uploaded graph's labels took
+      if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+      *v = c->g.cc_passes;
+      return BCMI_OK;
+    }
+    if (k == "block") *v = p.block;
+    else if (k == "grid") *v = p.grid;
+    else if (k == "grid_plain") *v = p.grid_plain;
+    else if (k == "grid_back") *v = p.grid_back;
+    else if (k == "chunk") *v = p.chunk;
+    else if (k == "alpha_plain") *v = p.alpha_plain;
+    else if (k == "alpha_batched") *v = p.alpha_batched;
+    else if (k == "beta_plain") *v = p.beta_plain;
+    else if (k == "beta_batched") *v = p.beta_batched;
+    else if (k == "gamma_push") *v = p.gamma_push;
+    else if (k == "gamma_back") *v = p.gamma_back;
+    else if (k == "narrow_sigma") *v = p.narrow_sigma;
+    else if (k == "pack_bu") *v = p.pack_bu;
+    else if (k == "fuse_bu") *v = p.fuse_bu;
+    else if (k == "steps") *v = p.steps;
+    else if (k == "width") *v = p.width;
+    else if (k == "ktime") *v = p.ktime;
+    else if (k == "trace") *v = p.trace;
+    else if (k == "graphs") *v = p.graphs;
+    else if (k == "reorder") *v = p.reorder;
+    else if (k == "batching") *v = p.batching;
+    else if (k == "kflags") *v = p.kflags;
+    else if (k == "sigma") *v = p.sigma;
+    else if (k == "sgrid") *v = p.sgrid;
+    else if (k == "sparse_verts") *v = p.sparse_verts;
+    else if (k == "streams") *v = p.streams;
+    else if (k == "strategy") *v = p.strategy;
+    else if (k == "wg_depth") *v = p.wg_depth;
+    else if (k == "wg_mem") *v = p.wg_mem;
+    else if (k == "wg_slots") *v = p.wg_slots;
+    else if (k == "deg2_slots") *v = p.deg2_slots;
+    else if (k == "concurrency") *v = p.concurrency;
+    else if (k == "deg2_mem") *v = p.deg2_mem;
+    else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
     return BCMI_OK;
-  }
-  if (k == "block") *v = p.block;
-  else if (k == "grid") *v = p.grid;
-  else if (k == "grid_plain") *v = p.grid_plain;
-  else if (k == "grid_back") *v = p.grid_back;
-  else if (k == "chunk") *v = p.chunk;
-  else if (k == "alpha_plain") *v = p.alpha_plain;
-  else if (k == "alpha_batched") *v = p.alpha_batched;
-  else if (k == "beta_plain") *v = p.beta_plain;
-  else if (k == "beta_batched") *v = p.beta_batched;
-  else if (k == "gamma_push") *v = p.gamma_push;
-  else if (k == "gamma_back") *v = p.gamma_back;
-  else if (k == "narrow_sigma") *v = p.narrow_sigma;
-  else if (k == "pack_bu") *v = p.pack_bu;
-  else if (k == "fuse_bu") *v = p.fuse_bu;
-  else if (k == "steps") *v = p.steps;
-  else if (k == "width") *v = p.width;
-  else if (k == "ktime") *v = p.ktime;
-  else if (k == "trace") *v = p.trace;
-  else if (k == "graphs") *v = p.graphs;
-  else if (k == "reorder") *v = p.reorder;
-  else if (k == "batching") *v = p.batching;
-  else if (k == "kflags") *v = p.kflags;
-  else if (k == "sigma") *v = p.sigma;
-  else if (k == "sgrid") *v = p.sgrid;
-  else if (k == "sparse_verts") *v = p.sparse_verts;
-  else if (k == "streams") *v = p.streams;
-  else if (k == "strategy") *v = p.strategy;
-  else if (k == "wg_depth") *v = p.wg_depth;
-  else if (k == "wg_mem") *v = p.wg_mem;
-  else if (k == "wg_slots") *v = p.wg_slots;
-  else if (k == "deg2_slots") *v = p.deg2_slots;
-  else if (k == "concurrency") *v = p.concurrency;
-  else if (k == "deg2_mem") *v = p.deg2_mem;
-  else return set_error(BCMI_ERR_ARG, "unknown parameter " + k);
-  return BCMI_OK;
+  });
 }
 
 int bcmi_upload_graph(bcmi_ctx* c, int64_t n, const int64_t* off, const int32_t* adj, int directed, int heuristic) {
-  CHECK_CTX(c);
-  if (!off || (!adj && n > 0 && off[n] > 0)) return set_error(BCMI_ERR_ARG, "null CSR arrays");
-  if (heuristic != BCMI_H_NONE && heuristic != BCMI_H_PEEL) return set_error(BCMI_ERR_ARG, "heuristic must be 0 or 1");
-  std::lock_guard<std::mutex> lk(c->mu);
-  int rc = use_device(c);
-  if (rc) return rc;
-  drop_workspaces(c);
-  if (c->bc) { (void)hipFree(c->bc); c->bc = nullptr; }
-  c->have_graph = false;
-  rc = devgraph_build(c->g, n, off, adj, directed, heuristic, c->prm, c->st);
-  if (rc) { c->g.release(); return rc; }
-  hipError_t e = hipMalloc((void**)&c->bc, sizeof(double) * n);
-  if (e != hipSuccess) { c->g.release(); return set_error(BCMI_ERR_NOMEM, "BC accumulator allocation failed"); }
-  e = hipMemsetAsync(c->bc, 0, sizeof(double) * n, c->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
-  c->have_graph = true;
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!off || (!adj && n > 0 && off[n] > 0)) return set_error(BCMI_ERR_ARG, "null CSR arrays");
+    if (heuristic != BCMI_H_NONE && heuristic != BCMI_H_PEEL) return set_error(BCMI_ERR_ARG, "heuristic must be 0 or 1");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = use_device(c);
+    if (rc) return rc;
+    drop_workspaces(c);
+    if (c->bc) { (void)hipFree(c->bc); c->bc = nullptr; }
+    c->have_graph = false;
+    rc = devgraph_build(c->g, n, off, adj, directed, heuristic, c->prm, c->st);
+    if (rc) { c->g.release(); return rc; }
+    hipError_t e = hipMalloc((void**)&c->bc, sizeof(double) * n);
+    if (e != hipSuccess) { c->g.release(); return set_error(BCMI_ERR_NOMEM, "BC accumulator allocation failed"); }
+    e = hipMemsetAsync(c->bc, 0, sizeof(double) * n, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+    c->have_graph = true;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_get_peel(bcmi_ctx* c, int32_t* round, int32_t* parent, uint64_t* w, int32_t* nrounds) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  use_device(c);
-  int64_t n = c->g.n;
-  hipError_t e = hipSuccess;
-  if (round && e == hipSuccess) e = hipMemcpy(round, c->g.round, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
-  if (parent && e == hipSuccess) e = hipMemcpy(parent, c->g.parent, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
-  if (w && e == hipSuccess) e = hipMemcpy(w, c->g.w, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
-  if (round) c->g.to_caller(round);
-  if (w) c->g.to_caller(w);
-  if (parent) {
-    c->g.to_caller(parent);
-    if (c->g.relabeled)
-      for (int64_t v = 0; v < n; v++)
-        if (parent[v] >= 0) parent[v] = c->g.h_n2o[parent[v]];
-  }
-  if (nrounds) *nrounds = c->g.nrounds;
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    use_device(c);
+    int64_t n = c->g.n;
+    hipError_t e = hipSuccess;
+    if (round && e == hipSuccess) e = hipMemcpy(round, c->g.round, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+    if (parent && e == hipSuccess) e = hipMemcpy(parent, c->g.parent, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+    if (w && e == hipSuccess) e = hipMemcpy(w, c->g.w, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+    if (round) c->g.to_caller(round);
+    if (w) c->g.to_caller(w);
+    if (parent) {
+      c->g.to_caller(parent);
+      if (c->g.relabeled)
+        for (int64_t v = 0; v < n; v++)
+          if (parent[v] >= 0) parent[v] = c->g.h_n2o[parent[v]];
+    }
+    if (nrounds) *nrounds = c->g.nrounds;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_get_labels(bcmi_ctx* c, int32_t* label) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  if (!label) return set_error(BCMI_ERR_ARG, "null output");
-  use_device(c);
-  hipError_t e = hipMemcpy(label, c->g.label, sizeof(int32_t) * c->g.n, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
-  if (c->g.relabeled) {
-    // internal labels are the smallest internal id; report the smallest caller id
-    int64_t n = c->g.n;
-    std::vector<int32_t> mn(n, INT32_MAX);
-    for (int64_t i = 0; i < n; i++) mn[label[i]] = std::min(mn[label[i]], c->g.h_n2o[i]);
-    std::vector<int32_t> out(n);
-    for (int64_t v = 0; v < n; v++) out[v] = mn[label[c->g.h_o2n[v]]];
-    memcpy(label, out.data(), sizeof(int32_t) * n);
-  }
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    if (!label) return set_error(BCMI_ERR_ARG, "null output");
+    use_device(c);
+    hipError_t e = hipMemcpy(label, c->g.label, sizeof(int32_t) * c->g.n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+    if (c->g.relabeled) {
+      // internal labels are the smallest internal id; report the smallest caller id
+      int64_t n = c->g.n;
+      std::vector<int32_t> mn(n, INT32_MAX);
+      for (int64_t i = 0; i < n; i++) mn[label[i]] = std::min(mn[label[i]], c->g.h_n2o[i]);
+      std::vector<int32_t> out(n);
+      for (int64_t v = 0; v < n; v++) out[v] = mn[label[c->g.h_o2n[v]]];
+      memcpy(label, out.data(), sizeof(int32_t) * n);
+    }
+    return BCMI_OK;
+  });
 }
 
 int bcmi_bc_reset(bcmi_ctx* c) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  use_device(c);
-  hipError_t e = hipMemsetAsync(c->bc, 0, sizeof(double) * c->g.n, c->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    use_device(c);
+    hipError_t e = hipMemsetAsync(c->bc, 0, sizeof(double) * c->g.n, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+    return BCMI_OK;
+  });
 }
 
 int bcmi_bc_run(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int mode, int src_flags, bcmi_run_stats* st) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  if (nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad source list");
-  if (mode != BCMI_MODE_PLAIN && mode != BCMI_MODE_BATCHED) return set_error(BCMI_ERR_ARG, "bad mode");
-  if (src_flags & ~(BCMI_SRC_EXPAND | BCMI_SRC_DEG2)) return set_error(BCMI_ERR_ARG, "bad source flags");
-  std::lock_guard<std::mutex> lk(c->mu);
-  int rc = use_device(c);
-  if (rc) return rc;
-  Workspace* ws = nullptr;
-  rc = get_ws(c, mode, &ws);
-  if (rc) return rc;
-  RunResult R;
-  rc = bc_run(c->g, ws, c->prm, sources, nsrc, mode, src_flags, &R);
-  c->last = R;
-  if (st) {
-    memset(st, 0, sizeof(*st));
-    st->sources_run = R.sweeps;
-    st->batches = R.launches;
-    st->te = R.te;
-    st->bytes = R.bytes;
-    st->gpu_ms = R.gpu_ms;
-    st->wall_ms = R.wall_ms;
-    st->max_levels = R.max_levels;
-    st->sum_n = R.sum_n;
-    st->sum_m = R.sum_m;
-    st->sum_a = R.sum_a;
-    st->sum_pairs = R.pairs;
-    st->scaled_batches = R.scaled_batches;
-    st->wg_sources = R.wg_sources;
-    st->deg2_sources = R.deg2_sources;
-    for (int64_t u : R.uni_n) st->sum_union += u;
-  }
-  return rc;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    if (nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad source list");
+    if (mode != BCMI_MODE_PLAIN && mode != BCMI_MODE_BATCHED) return set_error(BCMI_ERR_ARG, "bad mode");
+    if (src_flags & ~(BCMI_SRC_EXPAND | BCMI_SRC_DEG2)) return set_error(BCMI_ERR_ARG, "bad source flags");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = use_device(c);
+    if (rc) return rc;
+    Workspace* ws = nullptr;
+    rc = get_ws(c, mode, &ws);
+    if (rc) return rc;
+    RunResult R;
+    rc = bc_run(c->g, ws, c->prm, sources, nsrc, mode, src_flags, &R);
+    // a run stopped part way (device or host failure) may leave a sweep's
+    // state uncleared: the next call starts from fresh workspaces
+    if (rc == BCMI_ERR_HIP || rc == BCMI_ERR_NOMEM) drop_workspaces(c);
+    c->last = R;
+    if (st) {
+      memset(st, 0, sizeof(*st));
+      st->sources_run = R.sweeps;
+      st->batches = R.launches;
+      st->te = R.te;
+      st->bytes = R.bytes;
+      st->gpu_ms = R.gpu_ms;
+      st->wall_ms = R.wall_ms;
+      st->max_levels = R.max_levels;
+      st->sum_n = R.sum_n;
+      st->sum_m = R.sum_m;
+      st->sum_a = R.sum_a;
+      st->sum_pairs = R.pairs;
+      st->scaled_batches = R.scaled_batches;
+      st->wg_sources = R.wg_sources;
+      st->deg2_sources = R.deg2_sources;
+      for (int64_t u : R.uni_n) st->sum_union += u;
+    }
+    return rc;
+  });
 }
 
 int bcmi_bc_run_static(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, bcmi_run_stats* st) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  if (nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad source list");
-  std::lock_guard<std::mutex> lk(c->mu);
-  int rc = use_device(c);
-  if (rc) return rc;
-  StaticRun S;
-  rc = bc_run_static(c->g, c->bc, c->st, c->device, c->prm, sources, nsrc, &S);
-  // the per-sweep records describe bcmi_bc_run's last run: drop them (clear keeps
-  // the vectors' storage, so nothing is freed or allocated)
-  c->last.n_s.clear(); c->last.m_s.clear(); c->last.a_s.clear(); c->last.levels.clear(); c->last.t_s.clear();
-  c->last.ktime_ms = 0; c->last.klaunches = 0;
-  if (st) {
-    memset(st, 0, sizeof(*st));
-    st->sources_run = S.sweeps;
-    st->batches = S.sweeps;
-    st->te = S.te;
-    st->bytes = S.bytes;
-    st->gpu_ms = S.wall_ms;
-    st->wall_ms = S.wall_ms;
-    st->max_levels = S.max_levels;
-    st->sum_n = S.sum_n;
-    st->sum_m = S.sum_m;
-    st->sum_a = S.sum_a;
-  }
-  return rc;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    if (nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad source list");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = use_device(c);
+    if (rc) return rc;
+    StaticRun S;
+    rc = bc_run_static(c->g, c->bc, c->st, c->device, c->prm, sources, nsrc, &S);
+    // the per-sweep records describe bcmi_bc_run's last run: drop them (clear keeps
+    // the vectors' storage, so nothing is freed or allocated)
+    c->last.n_s.clear(); c->last.m_s.clear(); c->last.a_s.clear(); c->last.levels.clear(); c->last.t_s.clear();
+    c->last.ktime_ms = 0; c->last.klaunches = 0;
+    if (st) {
+      memset(st, 0, sizeof(*st));
+      st->sources_run = S.sweeps;
+      st->batches = S.sweeps;
+      st->te = S.te;
+      st->bytes = S.bytes;
+      st->gpu_ms = S.wall_ms;
+      st->wall_ms = S.wall_ms;
+      st->max_levels = S.max_levels;
+      st->sum_n = S.sum_n;
+      st->sum_m = S.sum_m;
+      st->sum_a = S.sum_a;
+    }
+    return rc;
+  });
 }
 
 int bcmi_batch_order(bcmi_ctx* c, const int32_t* sources, int64_t nsrc, int32_t* out) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  if (nsrc < 0 || (nsrc > 0 && (!sources || !out))) return set_error(BCMI_ERR_ARG, "bad source list");
-  std::lock_guard<std::mutex> lk(c->mu);
-  return batch_order(c->g, c->prm, sources, nsrc, out);
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    if (nsrc < 0 || (nsrc > 0 && (!sources || !out))) return set_error(BCMI_ERR_ARG, "bad source list");
+    std::lock_guard<std::mutex> lk(c->mu);
+    return batch_order(c->g, c->prm, sources, nsrc, out);
+  });
 }
 
 int bcmi_get_bc(bcmi_ctx* c, double* bc) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  if (!bc) return set_error(BCMI_ERR_ARG, "null output");
-  use_device(c);
-  hipError_t e = hipMemcpy(bc, c->bc, sizeof(double) * c->g.n, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
-  c->g.to_caller(bc);
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    if (!bc) return set_error(BCMI_ERR_ARG, "null output");
+    use_device(c);
+    hipError_t e = hipMemcpy(bc, c->bc, sizeof(double) * c->g.n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_error(BCMI_ERR_HIP, hipGetErrorString(e));
+    c->g.to_caller(bc);
+    return BCMI_OK;
+  });
 }
 
 int bcmi_get_sweep_stats(bcmi_ctx* c, int64_t* n_s, int64_t* m_s, int64_t* a_s, int32_t* levels) {
-  CHECK_CTX(c);
-  const RunResult& R = c->last;
-  size_t k = R.n_s.size();
-  if (n_s) memcpy(n_s, R.n_s.data(), k * sizeof(int64_t));
-  if (m_s) memcpy(m_s, R.m_s.data(), k * sizeof(int64_t));
-  if (a_s) memcpy(a_s, R.a_s.data(), k * sizeof(int64_t));
-  if (levels) memcpy(levels, R.levels.data(), R.levels.size() * sizeof(int32_t));
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    const RunResult& R = c->last;
+    size_t k = R.n_s.size();
+    if (n_s) memcpy(n_s, R.n_s.data(), k * sizeof(int64_t));
+    if (m_s) memcpy(m_s, R.m_s.data(), k * sizeof(int64_t));
+    if (a_s) memcpy(a_s, R.a_s.data(), k * sizeof(int64_t));
+    if (levels) memcpy(levels, R.levels.data(), R.levels.size() * sizeof(int32_t));
+    return BCMI_OK;
+  });
 }
 
 int bcmi_get_sweep_times(bcmi_ctx* c, double* ms) {
-  CHECK_CTX(c);
-  if (!ms) return set_error(BCMI_ERR_ARG, "null output");
-  const RunResult& R = c->last;
-  if (!R.t_s.empty()) memcpy(ms, R.t_s.data(), R.t_s.size() * sizeof(double));
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!ms) return set_error(BCMI_ERR_ARG, "null output");
+    const RunResult& R = c->last;
+    if (!R.t_s.empty()) memcpy(ms, R.t_s.data(), R.t_s.size() * sizeof(double));
+    return BCMI_OK;
+  });
 }
 
 int bcmi_get_kernel_time(bcmi_ctx* c, double* ms, int64_t* launches) {
-  CHECK_CTX(c);
-  if (ms) *ms = c->last.ktime_ms;
-  if (launches) *launches = c->last.klaunches;
-  return BCMI_OK;
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (ms) *ms = c->last.ktime_ms;
+    if (launches) *launches = c->last.klaunches;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_sssp(bcmi_ctx* c, int32_t s, int32_t* depth, uint64_t* sigma, double* delta) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  if (!depth || !sigma || !delta) return set_error(BCMI_ERR_ARG, "null output");
-  std::lock_guard<std::mutex> lk(c->mu);
-  use_device(c);
-  Workspace* ws = nullptr;
-  int rc = get_ws(c, BCMI_MODE_PLAIN, &ws);
-  if (rc) return rc;
-  return sssp_debug(c->g, ws, c->prm, s, depth, sigma, delta);
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    if (!depth || !sigma || !delta) return set_error(BCMI_ERR_ARG, "null output");
+    std::lock_guard<std::mutex> lk(c->mu);
+    use_device(c);
+    Workspace* ws = nullptr;
+    int rc = get_ws(c, BCMI_MODE_PLAIN, &ws);
+    if (rc) return rc;
+    return sssp_debug(c->g, ws, c->prm, s, depth, sigma, delta);
+  });
 }
 
 int bcmi_msbfs(bcmi_ctx* c, const int32_t* sources, int k, int32_t* depth, uint64_t* sigma, double* delta) {
-  CHECK_CTX(c);
-  if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
-  if (!sources || !depth || !sigma || !delta) return set_error(BCMI_ERR_ARG, "null argument");
-  std::lock_guard<std::mutex> lk(c->mu);
-  use_device(c);
-  Workspace* ws = nullptr;
-  int rc = get_ws(c, BCMI_MODE_BATCHED, &ws);
-  if (rc) return rc;
-  return msbfs_debug(c->g, ws, c->prm, sources, k, depth, sigma, delta);
+  return guard([&]() -> int {
+    CHECK_CTX(c);
+    if (!c->have_graph) return set_error(BCMI_ERR_STATE, "no graph uploaded");
+    if (!sources || !depth || !sigma || !delta) return set_error(BCMI_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    use_device(c);
+    Workspace* ws = nullptr;
+    int rc = get_ws(c, BCMI_MODE_BATCHED, &ws);
+    if (rc) return rc;
+    return msbfs_debug(c->g, ws, c->prm, sources, k, depth, sigma, delta);
+  });
 }
 
 }  // extern "C"

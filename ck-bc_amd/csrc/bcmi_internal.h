@@ -2,6 +2,8 @@
 // the query server.
 #pragma once
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -11,6 +13,22 @@ namespace bcmi {
 
 // Records the message for bcmi_last_error() (thread-local) and returns code.
 int set_error(int code, const std::string& msg);
+
+// Runs the body of a C entry point: a C++ exception from inside the library
+// (host memory for a container, or a thread that cannot start) returns
+// BCMI_ERR_NOMEM with its message instead of crossing the C ABI.
+template <class F>
+int guard(F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return set_error(BCMI_ERR_NOMEM, "out of host memory");
+  } catch (const std::exception& e) {
+    return set_error(BCMI_ERR_NOMEM, std::string("host resources exhausted: ") + e.what());
+  } catch (...) {
+    return set_error(BCMI_ERR_NOMEM, "host resources exhausted");
+  }
+}
 
 bcmi_graph* graph_alloc(int64_t n, int64_t m, int directed);
 int graph_save(const bcmi_graph* g, const char* path, uint64_t src_size, int64_t src_mtime_ns);

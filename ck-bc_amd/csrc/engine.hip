@@ -42,6 +42,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 
@@ -2998,8 +2999,10 @@ static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std
   std::vector<std::vector<uint8_t>> dep(H, std::vector<uint8_t>(n, 255));
   std::vector<int> hmax(H, 0);
   std::vector<std::thread> th;
-  for (int h = 0; h < (int)hubs.size(); h++)
-    th.emplace_back([&, h] {
+  th.reserve(hubs.size());
+  std::atomic<bool> hub_fail{false};
+  auto hub_bfs = [&](int h) {
+    try {
       // level-synchronous BFS: depths are stored saturated at 254 (255 =
       // unreached), the level count stays exact
       std::vector<uint8_t>& D = dep[h];
@@ -3021,8 +3024,20 @@ static void hub_signature(int64_t n, const int64_t* off, const int32_t* adj, std
         if (q.size() > hi) level++;
       }
       hmax[h] = level;
-    });
+    } catch (...) {
+      hub_fail = true;  // host memory: reported after the join
+    }
+  };
+  // one thread per hub; a hub whose thread cannot start is swept here
+  for (int h = 0; h < (int)hubs.size(); h++) {
+    try {
+      th.emplace_back(hub_bfs, h);
+    } catch (const std::system_error&) {
+      hub_bfs(h);
+    }
+  }
   for (auto& t : th) t.join();
+  if (hub_fail) throw std::bad_alloc();
   *max_depth = 0;
   for (int h = 0; h < std::min(HD, (int)hubs.size()); h++) *max_depth = std::max(*max_depth, hmax[h]);
   sig.assign(n, 0);
@@ -4646,7 +4661,7 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
   std::atomic<bool> stop{false};
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
-  auto work = [&](int t) {
+  auto work_body = [&](int t) {
     RunPart& P = parts[t];
     if (t > 0 && hipSetDevice(dev) != hipSuccess) {
       P.rc = BCMI_ERR_HIP;
@@ -4684,9 +4699,29 @@ int bc_run(const DevGraph& g, Workspace* ws, const Params& prm, const int32_t* s
       }
     }
   };
+  // an exception in a sweep thread (host memory) stops the call with an
+  // error instead of ending the process
+  auto work = [&](int t) {
+    try {
+      work_body(t);
+    } catch (...) {
+      parts[t].rc = BCMI_ERR_NOMEM;
+      parts[t].msg = "out of host memory in a sweep thread";
+      stop = true;
+    }
+  };
   {
+    // a helper thread that cannot start leaves its share to the others (the
+    // work items come from the shared counter)
     std::vector<std::thread> th;
-    for (int t = 1; t < K; t++) th.emplace_back(work, t);
+    th.reserve(K);
+    for (int t = 1; t < K; t++) {
+      try {
+        th.emplace_back(work, t);
+      } catch (const std::system_error&) {
+        break;
+      }
+    }
     work(0);
     for (auto& x : th) x.join();
   }

@@ -557,52 +557,68 @@ extern "C" {
 
 int bcmi_graph_read_edgelist(const char* path, uint64_t n, int directed, bcmi_graph** out,
                              uint64_t* bad_line, uint64_t* hdr_nodes, uint64_t* hdr_edges) {
-  if (!path || !out) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
-  return bcmi::read_edgelist(path, n, directed, out, bad_line, hdr_nodes, hdr_edges);
+  return bcmi::guard([&]() -> int {
+    if (!path || !out) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
+    return bcmi::read_edgelist(path, n, directed, out, bad_line, hdr_nodes, hdr_edges);
+  });
 }
 
 int bcmi_graph_save(const bcmi_graph* g, const char* path, uint64_t src_size, int64_t src_mtime_ns) {
-  if (!g || !path) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
-  return bcmi::graph_save(g, path, src_size, src_mtime_ns);
+  return bcmi::guard([&]() -> int {
+    if (!g || !path) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
+    return bcmi::graph_save(g, path, src_size, src_mtime_ns);
+  });
 }
 
 int bcmi_graph_load(const char* path, bcmi_graph** out, uint64_t* src_size, int64_t* src_mtime_ns) {
-  if (!path || !out) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
-  return bcmi::graph_load(path, out, src_size, src_mtime_ns);
+  return bcmi::guard([&]() -> int {
+    if (!path || !out) return bcmi::set_error(BCMI_ERR_ARG, "null argument");
+    return bcmi::graph_load(path, out, src_size, src_mtime_ns);
+  });
 }
 
 int bcmi_graph_from_edges(const int32_t* tails, const int32_t* heads, int64_t k, int64_t n, int directed,
                           bcmi_graph** out) {
-  if ((!tails || !heads) && k > 0) return bcmi::set_error(BCMI_ERR_ARG, "null edge arrays");
-  if (!out || n < 1 || n > INT32_MAX) return bcmi::set_error(BCMI_ERR_ARG, "bad vertex count");
-  for (int64_t i = 0; i < k; i++)
-    if (tails[i] < 0 || tails[i] >= n || heads[i] < 0 || heads[i] >= n)
-      return bcmi::set_error(BCMI_ERR_RANGE, "edge endpoint out of range at index " + std::to_string(i));
-  bcmi_graph* g = bcmi::csr_from_arcs(n, tails, heads, k, directed ? 0 : 1, directed);
-  if (!g) return bcmi::set_error(BCMI_ERR_NOMEM, "out of host memory");
-  *out = g;
-  return BCMI_OK;
+  return bcmi::guard([&]() -> int {
+    if ((!tails || !heads) && k > 0) return bcmi::set_error(BCMI_ERR_ARG, "null edge arrays");
+    if (!out || n < 1 || n > INT32_MAX) return bcmi::set_error(BCMI_ERR_ARG, "bad vertex count");
+    for (int64_t i = 0; i < k; i++)
+      if (tails[i] < 0 || tails[i] >= n || heads[i] < 0 || heads[i] >= n)
+        return bcmi::set_error(BCMI_ERR_RANGE, "edge endpoint out of range at index " + std::to_string(i));
+    bcmi_graph* g = bcmi::csr_from_arcs(n, tails, heads, k, directed ? 0 : 1, directed);
+    if (!g) return bcmi::set_error(BCMI_ERR_NOMEM, "out of host memory");
+    *out = g;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_gen_rmat(int scale, int edgefactor, double a, double b, double c, uint64_t seed, int permute,
                   bcmi_graph** out) {
-  if (!out) return bcmi::set_error(BCMI_ERR_ARG, "null output");
-  return bcmi::gen_rmat(scale, edgefactor, a, b, c, seed, permute, out);
+  return bcmi::guard([&]() -> int {
+    if (!out) return bcmi::set_error(BCMI_ERR_ARG, "null output");
+    return bcmi::gen_rmat(scale, edgefactor, a, b, c, seed, permute, out);
+  });
 }
 
 int bcmi_gen_pendant_trees(const bcmi_graph* core, double frac, uint64_t seed, bcmi_graph** out) {
-  if (!out) return bcmi::set_error(BCMI_ERR_ARG, "null output");
-  return bcmi::gen_pendant_trees(core, frac, seed, out);
+  return bcmi::guard([&]() -> int {
+    if (!out) return bcmi::set_error(BCMI_ERR_ARG, "null output");
+    return bcmi::gen_pendant_trees(core, frac, seed, out);
+  });
 }
 
 int bcmi_gen_block_chain(int64_t nblocks, int block, double avgdeg, uint64_t seed, bcmi_graph** out) {
-  if (!out) return bcmi::set_error(BCMI_ERR_ARG, "null output");
-  return bcmi::gen_block_chain(nblocks, block, avgdeg, seed, out);
+  return bcmi::guard([&]() -> int {
+    if (!out) return bcmi::set_error(BCMI_ERR_ARG, "null output");
+    return bcmi::gen_block_chain(nblocks, block, avgdeg, seed, out);
+  });
 }
 
 int bcmi_sample_sources(const bcmi_graph* g, int64_t k, uint64_t seed, int32_t* out) {
-  if (!g || (!out && k > 0) || k < 0) return bcmi::set_error(BCMI_ERR_ARG, "bad argument");
-  return bcmi::sample_sources(g, k, seed, out);
+  return bcmi::guard([&]() -> int {
+    if (!g || (!out && k > 0) || k < 0) return bcmi::set_error(BCMI_ERR_ARG, "bad argument");
+    return bcmi::sample_sources(g, k, seed, out);
+  });
 }
 
 void bcmi_graph_free(bcmi_graph* g) {

@@ -232,124 +232,151 @@ static void worker(bcmi_server* s) {
 extern "C" {
 
 int bcmi_server_create(int device, int64_t hbm_budget, int max_slots, bcmi_server** out) {
-  if (!out || hbm_budget <= 0 || max_slots < 1 || max_slots > 64) return set_error(BCMI_ERR_ARG, "bad server arguments");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
-    return set_error(BCMI_ERR_HIP, "no such HIP device");
-  if (hipSetDevice(device) != hipSuccess) return set_error(BCMI_ERR_HIP, "hipSetDevice failed");
-  bcmi_server* s = new bcmi_server();
-  s->device = device;
-  s->budget = hbm_budget;
-  s->max_slots = max_slots;
-  s->t0 = now_ms();
-  // queries already overlap on the slots: each runs its batches one at a time
-  // (no helper workspaces outside the budget's accounting of one per slot)
-  s->prm.concurrency = 1;
-  s->sched = std::thread(scheduler, s);
-  for (int i = 0; i < max_slots; i++) s->workers.emplace_back(worker, s);
-  *out = s;
-  return BCMI_OK;
+  return guard([&]() -> int {
+    if (!out || hbm_budget <= 0 || max_slots < 1 || max_slots > 64) return set_error(BCMI_ERR_ARG, "bad server arguments");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+      return set_error(BCMI_ERR_HIP, "no such HIP device");
+    if (hipSetDevice(device) != hipSuccess) return set_error(BCMI_ERR_HIP, "hipSetDevice failed");
+    bcmi_server* s = new bcmi_server();
+    s->device = device;
+    s->budget = hbm_budget;
+    s->max_slots = max_slots;
+    s->t0 = now_ms();
+    // queries already overlap on the slots: each runs its batches one at a time
+    // (no helper workspaces outside the budget's accounting of one per slot)
+    s->prm.concurrency = 1;
+    try {
+      s->workers.reserve(max_slots);
+      s->sched = std::thread(scheduler, s);
+      for (int i = 0; i < max_slots; i++) s->workers.emplace_back(worker, s);
+    } catch (const std::exception& e) {
+      // a thread that cannot start: stop the ones that did, then report
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+      }
+      s->cv_sched.notify_all();
+      s->cv_work.notify_all();
+      if (s->sched.joinable()) s->sched.join();
+      for (auto& t : s->workers) t.join();
+      delete s;
+      return set_error(BCMI_ERR_NOMEM, std::string("server threads could not start: ") + e.what());
+    }
+    *out = s;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_server_destroy(bcmi_server* s) {
-  if (!s) return set_error(BCMI_ERR_ARG, "null server");
-  {
-    std::unique_lock<std::mutex> lk(s->mu);
-    // finish what was admitted; drop what is still queued
-    s->cv_done.wait(lk, [&] { return s->assigned.empty() && active_slots(s) == 0; });
-    s->stop = true;
-    // queries never admitted complete with an error (their waiters return)
-    for (Query* q : s->pending) {
-      q->status = BCMI_ERR_STATE;
-      q->err = "server destroyed before the query was admitted";
-      q->done = true;
+  return guard([&]() -> int {
+    if (!s) return set_error(BCMI_ERR_ARG, "null server");
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      // finish what was admitted; drop what is still queued
+      s->cv_done.wait(lk, [&] { return s->assigned.empty() && active_slots(s) == 0; });
+      s->stop = true;
+      // queries never admitted complete with an error (their waiters return)
+      for (Query* q : s->pending) {
+        q->status = BCMI_ERR_STATE;
+        q->err = "server destroyed before the query was admitted";
+        q->done = true;
+      }
+      s->pending.clear();
+      s->cv_done.notify_all();
     }
-    s->pending.clear();
-    s->cv_done.notify_all();
-  }
-  s->cv_sched.notify_all();
-  s->cv_work.notify_all();
-  s->sched.join();
-  for (auto& t : s->workers) t.join();
-  (void)hipSetDevice(s->device);
-  for (auto& sl : s->slots) workspace_destroy(sl->ws);
-  for (auto& g : s->graphs) g->release();
-  delete s;
-  return BCMI_OK;
+    s->cv_sched.notify_all();
+    s->cv_work.notify_all();
+    s->sched.join();
+    for (auto& t : s->workers) t.join();
+    (void)hipSetDevice(s->device);
+    for (auto& sl : s->slots) workspace_destroy(sl->ws);
+    for (auto& g : s->graphs) g->release();
+    delete s;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_server_add_graph(bcmi_server* s, int64_t n, const int64_t* off, const int32_t* adj, int32_t* graph_id) {
-  if (!s || !off || !graph_id) return set_error(BCMI_ERR_ARG, "null argument");
-  (void)hipSetDevice(s->device);
-  std::unique_ptr<DevGraph> g(new DevGraph());
-  hipStream_t st;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return set_error(BCMI_ERR_HIP, "stream");
-  int rc = devgraph_build(*g, n, off, adj, 0, BCMI_H_NONE, s->prm, st);
-  (void)hipStreamDestroy(st);
-  if (rc != BCMI_OK) return rc;
-  std::lock_guard<std::mutex> lk(s->mu);
-  if (s->graph_bytes + (int64_t)g->bytes > s->budget) {
-    g->release();
-    return set_error(BCMI_ERR_NOMEM, "graph does not fit in the HBM budget");
-  }
-  s->graph_bytes += (int64_t)g->bytes;
-  s->graph_ws_bytes.push_back(workspace_bytes(*g, BCMI_MODE_BATCHED, 64, s->prm));
-  s->graphs.push_back(std::move(g));
-  *graph_id = (int32_t)s->graphs.size() - 1;
-  return BCMI_OK;
+  return guard([&]() -> int {
+    if (!s || !off || !graph_id) return set_error(BCMI_ERR_ARG, "null argument");
+    (void)hipSetDevice(s->device);
+    std::unique_ptr<DevGraph> g(new DevGraph());
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return set_error(BCMI_ERR_HIP, "stream");
+    int rc = devgraph_build(*g, n, off, adj, 0, BCMI_H_NONE, s->prm, st);
+    (void)hipStreamDestroy(st);
+    if (rc != BCMI_OK) return rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->graph_bytes + (int64_t)g->bytes > s->budget) {
+      g->release();
+      return set_error(BCMI_ERR_NOMEM, "graph does not fit in the HBM budget");
+    }
+    s->graph_bytes += (int64_t)g->bytes;
+    s->graph_ws_bytes.push_back(workspace_bytes(*g, BCMI_MODE_BATCHED, 64, s->prm));
+    s->graphs.push_back(std::move(g));
+    *graph_id = (int32_t)s->graphs.size() - 1;
+    return BCMI_OK;
+  });
 }
 
 int bcmi_server_submit(bcmi_server* s, int32_t graph_id, const int32_t* sources, int64_t nsrc, int64_t* query_id) {
-  if (!s || !query_id || nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad argument");
-  std::unique_ptr<Query> q(new Query());
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (graph_id < 0 || graph_id >= (int32_t)s->graphs.size()) return set_error(BCMI_ERR_ARG, "unknown graph id");
-    int64_t n = s->graphs[graph_id]->n;
-    for (int64_t i = 0; i < nsrc; i++)
-      if (sources[i] < 0 || sources[i] >= n) return set_error(BCMI_ERR_RANGE, "source id out of range");
-    q->id = s->next_id++;
-    q->gid = graph_id;
-    q->src.assign(sources, sources + nsrc);
-    q->arrive = now_ms() - s->t0;
-    *query_id = q->id;
-    s->pending.push_back(q.get());
-    s->queries[q->id] = std::move(q);
-  }
-  s->cv_sched.notify_one();
-  return BCMI_OK;
+  return guard([&]() -> int {
+    if (!s || !query_id || nsrc < 0 || (nsrc > 0 && !sources)) return set_error(BCMI_ERR_ARG, "bad argument");
+    std::unique_ptr<Query> q(new Query());
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (graph_id < 0 || graph_id >= (int32_t)s->graphs.size()) return set_error(BCMI_ERR_ARG, "unknown graph id");
+      int64_t n = s->graphs[graph_id]->n;
+      for (int64_t i = 0; i < nsrc; i++)
+        if (sources[i] < 0 || sources[i] >= n) return set_error(BCMI_ERR_RANGE, "source id out of range");
+      q->id = s->next_id++;
+      q->gid = graph_id;
+      q->src.assign(sources, sources + nsrc);
+      q->arrive = now_ms() - s->t0;
+      *query_id = q->id;
+      s->pending.push_back(q.get());
+      s->queries[q->id] = std::move(q);
+    }
+    s->cv_sched.notify_one();
+    return BCMI_OK;
+  });
 }
 
 int bcmi_server_wait(bcmi_server* s, int64_t query_id, double* bc, bcmi_query_result* res) {
-  if (!s) return set_error(BCMI_ERR_ARG, "null server");
-  std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->queries.find(query_id);
-  if (it == s->queries.end()) return set_error(BCMI_ERR_ARG, "unknown query id");
-  Query* q = it->second.get();
-  s->cv_done.wait(lk, [&] { return q->done; });
-  if (res) {
-    res->query_id = q->id;
-    res->graph_id = q->gid;
-    res->status = q->status;
-    res->arrive_ms = q->arrive;
-    res->start_ms = q->start;
-    res->finish_ms = q->finish;
-    res->te = q->te;
-  }
-  int rc = q->status;
-  if (rc == BCMI_OK && bc) memcpy(bc, q->bc.data(), sizeof(double) * q->bc.size());
-  if (rc != BCMI_OK) set_error(rc, q->err);
-  s->queries.erase(it);
-  return rc;
+  return guard([&]() -> int {
+    if (!s) return set_error(BCMI_ERR_ARG, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->queries.find(query_id);
+    if (it == s->queries.end()) return set_error(BCMI_ERR_ARG, "unknown query id");
+    Query* q = it->second.get();
+    s->cv_done.wait(lk, [&] { return q->done; });
+    if (res) {
+      res->query_id = q->id;
+      res->graph_id = q->gid;
+      res->status = q->status;
+      res->arrive_ms = q->arrive;
+      res->start_ms = q->start;
+      res->finish_ms = q->finish;
+      res->te = q->te;
+    }
+    int rc = q->status;
+    if (rc == BCMI_OK && bc) memcpy(bc, q->bc.data(), sizeof(double) * q->bc.size());
+    if (rc != BCMI_OK) set_error(rc, q->err);
+    s->queries.erase(it);
+    return rc;
+  });
 }
 
 int bcmi_server_stats(bcmi_server* s, int64_t* completed, int64_t* peak_slots, int64_t* peak_bytes) {
-  if (!s) return set_error(BCMI_ERR_ARG, "null server");
-  std::lock_guard<std::mutex> lk(s->mu);
-  if (completed) *completed = s->completed;
-  if (peak_slots) *peak_slots = s->peak_active;
-  if (peak_bytes) *peak_bytes = s->peak_bytes;
-  return BCMI_OK;
+  return guard([&]() -> int {
+    if (!s) return set_error(BCMI_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (completed) *completed = s->completed;
+    if (peak_slots) *peak_slots = s->peak_active;
+    if (peak_bytes) *peak_bytes = s->peak_bytes;
+    return BCMI_OK;
+  });
 }
 
 }  // extern "C"
