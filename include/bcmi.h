@@ -27,7 +27,7 @@ extern "C" {
 #define BCMI_OK 0
 #define BCMI_ERR_ARG 1      /* invalid argument */
 #define BCMI_ERR_HIP 2      /* HIP runtime failure */
-#define BCMI_ERR_NOMEM 3    /* device or host allocation failed / over budget */
+#define BCMI_ERR_NOMEM 3    /* device or host allocation failed, host threads unavailable, or over budget */
 #define BCMI_ERR_OVERFLOW 4 /* a shortest-path count exceeded 2^64-1 */
 #define BCMI_ERR_STATE 5    /* call made in the wrong state (e.g. no graph) */
 #define BCMI_ERR_IO 6       /* file could not be read or parsed */
